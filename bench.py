@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: Siddon forward + adjoint Grays/s at 512x512x256, 1024x768, 720 views.
+
+One step = one exact Siddon forward projection of the seeded 30-ellipsoid
+phantom over all 720 views, then the exact adjoint of that sinogram, with
+every input already resident on the GPU.  Prints one JSON line on stdout
+(progress goes to stderr).
+
+Roofline numerator (algorithmic bytes per launch): sum over rays of
+(4 * N_int + 4), N_int = voxels with nonzero intersection length on the ray,
+counted by the GPU count kernel (pinned against the FP64 oracle in the tests).
+Peak: 8.0 TB/s HBM3E.
+
+cpu_baseline: the FP32 restatement of the reference algorithm
+(oracle/oracle.c, reference-mode flags) on the host cores, on a strided
+sample of the same workload (every 45th view = 16 views).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ct-nef_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, img_cpu, stride):
+    from oracle import oracle as O
+    import numpy as np
+    ov = O.vol_struct(wl.vol.grid, wl.vol.center, wl.vol.size)
+    d = wl.det
+    od = O.det_struct(d.cyl, d.SID, d.SAD, d.na, d.da, d.ai, d.nb, d.db, d.bi)
+    ang = wl.angles()
+    img = img_cpu.numpy()
+    nsel = len(range(0, wl.nv, stride))
+    rays = nsel * d.na * d.nb
+    t0 = time.perf_counter()
+    p = O.ref_fwd(ov, od, ang, img, flags=O.REF_FWD_FLAGS, v0=0, vstep=stride)
+    t1 = time.perf_counter()
+    O.ref_bp(ov, od, ang, p, flags=O.REF_BP_FLAGS, v0=0, vstep=stride)
+    t2 = time.perf_counter()
+    return {
+        "value": rays / (t2 - t0) / 1e9,
+        "unit": "Grays/s",
+        "cores": O.threads(),
+        "kind": "port",
+        "sample": f"reference-mode forward+adjoint restated in C (oracle/oracle.c), views 0::{stride} "
+                  f"({nsel} of {wl.nv}) of the same workload, {rays} rays; fwd {t1 - t0:.2f} s, adj {t2 - t1:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-stride", type=int, default=45)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(views would shard across GPUs with one all-reduce of the adjoint volume; not built)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nefct_hip as nh
+    from nefct_hip.configs import c2
+    from nefct_hip.phantom import ellipsoid_phantom
+
+    wl = c2()
+    vol, det = wl.vol, wl.det
+    dev = torch.device("cuda:0")
+    log(f"[bench] workload {wl.name}: building phantom")
+    img = ellipsoid_phantom(vol, wl.seed, wl.ellipsoids, device=dev)
+    ang = torch.tensor(wl.angles(), device=dev)
+    rays = wl.rays
+
+    log("[bench] counting intersections")
+    counts = nh.count_intersections(vol, det, ang, device=dev)
+    n_int = int(counts.sum(dtype=torch.int64).item())
+    del counts
+    bytes_fwd = 4 * n_int + 4 * rays
+    bytes_adj = 4 * rays + 4 * n_int
+
+    def step():
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e2 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sino = nh.project(img, vol, det, ang)
+        e1.record()
+        back = nh.backproject(sino, vol, det, ang)
+        e2.record()
+        return e0, e1, e2, sino, back
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs = []
+    for k in range(args.steps):
+        e0, e1, e2, sino, back = step()
+        evs.append((e0, e1, e2))
+        del sino, back
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    total_s = t1 - t0
+    t_fwd = sum(e0.elapsed_time(e1) for e0, e1, _ in evs) / 1e3 / args.steps
+    t_adj = sum(e1.elapsed_time(e2) for _, e1, e2 in evs) / 1e3 / args.steps
+    log(f"[bench] timed {args.steps} steps: {total_s:.3f} s (fwd {t_fwd * 1e3:.1f} ms, adj {t_adj * 1e3:.1f} ms)")
+
+    value = rays * args.steps / total_s / 1e9
+    dominant = "siddon_adjoint" if t_adj > t_fwd else "siddon_forward"
+    t_dom = max(t_adj, t_fwd)
+    b_dom = bytes_adj if t_adj > t_fwd else bytes_fwd
+    achieved = b_dom / t_dom / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline on views 0::{args.cpu_stride}")
+        cpu = cpu_baseline(wl, img.cpu(), args.cpu_stride)
+        log(f"[bench] CPU baseline {cpu['value']:.4f} Grays/s on {cpu['cores']} threads")
+
+    rec = {
+        "metric": "Siddon fwd+adjoint Grays/s at 512x512x256, 720 views",
+        "value": round(value, 5),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 30-ellipsoid phantom (numpy PCG64 seed 2), exact Siddon pair",
+        "config": {
+            "workload": wl.name,
+            "volume": list(vol.grid), "voxel_mm": vol.voxel[0],
+            "detector": [det.na, det.nb], "pitch_mm": det.da, "views": wl.nv,
+            "sod_mm": det.SID, "sdd_mm": det.SAD, "rays": rays, "sum_n_int": n_int,
+            "parallelism": "single GPU",
+        },
+        "fwd_grays_s": round(rays / t_fwd / 1e9, 5),
+        "adj_grays_s": round(rays / t_adj / 1e9, 5),
+        "voxel_intersections_per_s": (2 * n_int) / (t_fwd + t_adj),
+        "pct_hbm_peak_pair": round(100 * (bytes_fwd + bytes_adj) / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBS, 2),
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": b_dom,
+            "avg_launch_ms": round(t_dom * 1e3, 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,911 @@
+// Cone-beam Siddon projector pair for gfx950.
+//
+// Replaces nefct/cc/siddon_3d/siddon.cu.cc (project_device :12-321,
+// ProjectFlat/CyliKernel :323-390, backproject_device :450-764,
+// BackProjectFlat/CyliKernel :766-832, launchers :392-448, :834-890).
+//
+// Design (one lane = one ray, ray-driven, slab-synchronous):
+//  * per-ray set-up in FP64: end points, dominant axis, transverse slopes,
+//    entry/exit slab range clipped to the volume box, mm length per slab;
+//  * the slab loop runs in FP32 over wave-uniform slab indices (min/max over
+//    the wave's rays), so at every iteration the 64 lanes of a wavefront sit
+//    in the same slab;
+//  * the volume is re-laid out per call so that the transverse axis that
+//    varies across detector columns is the fastest one: x-slab walks read a
+//    y-fastest copy, y-slab walks an x-fastest copy.  Both copies carry a
+//    2-voxel zero border on their transverse axes so the slab kernel needs
+//    no per-voxel bounds test;
+//  * every volume access goes through a raw buffer resource sized to the
+//    copy, so an index error reads 0 / drops the atomic instead of faulting;
+//  * the adjoint scatters with buffer float atomics into the same two
+//    layouts, then one pass sums them into the x-fastest output.
+#include <cmath>
+#include <initializer_list>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "common.h"
+
+namespace {
+
+constexpr int PAD = 2;          // zero border of the transverse axes
+constexpr int BX = 64;          // detector columns per wavefront
+constexpr int BY = 4;           // detector rows per workgroup (one per wave)
+
+struct VolDev {
+    int n[3];
+    double lo[3];
+    double vs[3];
+};
+
+struct DetDev {
+    int cyl;
+    double sid, sad;
+    int na, nb;
+    double da, ai, db, bi;
+};
+
+// Element offsets of a padded layout for one walk direction.
+//   elem(ca, cb, t) = ca*sa + cb*sb + t*st + org   (ca, cb centred cells)
+struct Walk {
+    int sa, sb, st, org;
+    int na, nb;      // transverse extents (for checked walks / counting)
+    int ha, hb;      // centring offsets: true cell = centred + h
+};
+
+struct Layouts {
+    float* xf;       // (nx+2P) x ny x (nz+2P), x fastest
+    float* yf;       // (ny+2P) x nx x (nz+2P), y fastest
+    uint32_t xf_bytes, yf_bytes;
+    int NXP, NYP, NZP;
+};
+
+struct Ray {
+    int dom;             // 0: x slabs, 1: y slabs, 2: z slabs, -1: misses
+    int s0, s1;          // slab range [s0, s1)
+    int tc;              // centre slab
+    float ka, kb;        // transverse slopes (cells per slab)
+    float Ac, Bc;        // centred transverse coordinates at slab face tc
+    float inv_ka, inv_kb;
+    float pa, pb;        // 1 if slope > 0 else 0
+    float scale;         // mm per unit slab step (x 1/L^2 with NEFCT_F_INV_L2)
+};
+
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+__device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, int ia, int ib,
+                                           double S[3], double D[3])
+{
+    const double a = (g.ai + ia) * g.da - g.na * g.da * 0.5;
+    const double b = (g.bi + ib) * g.db - g.nb * g.db * 0.5;
+    double lx, ly;
+    if (g.cyl) {
+        lx = g.sad * cos(a) - g.sid;
+        ly = g.sad * sin(a);
+    } else {
+        lx = g.sad - g.sid;
+        ly = a;
+    }
+    S[0] = -g.sid * c;
+    S[1] = -g.sid * s;
+    S[2] = 0.0;
+    D[0] = lx * c - ly * s;
+    D[1] = lx * s + ly * c;
+    D[2] = b;
+}
+
+// Restricts [tin, tout] to the slab parameters where lo <= A + k t <= hi.
+__device__ __forceinline__ void clip_axis(double k, double A, double lo, double hi, double& tin, double& tout)
+{
+    if (k == 0.0) {
+        if (!(A > lo && A < hi)) { tin = 1.0; tout = 0.0; }
+        return;
+    }
+    double t0 = (lo - A) / k, t1 = (hi - A) / k;
+    if (t0 > t1) { double t = t0; t0 = t1; t1 = t; }
+    tin = fmax(tin, t0);
+    tout = fmin(tout, t1);
+}
+
+// FP64 ray set-up.  'ref' selects the reference's x/y-only dominance rule.
+__device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
+                          unsigned flags, bool adjoint, Ray& r)
+{
+    double S[3], D[3], uS[3], d[3];
+    ray_points(G, c, s, ia, ib, S, D);
+    for (int i = 0; i < 3; ++i) {
+        uS[i] = (S[i] - V.lo[i]) / V.vs[i];
+        d[i] = (D[i] - S[i]) / V.vs[i];
+    }
+    r.dom = -1;
+    r.s0 = r.s1 = 0;
+    r.tc = 0;
+    r.ka = r.kb = r.Ac = r.Bc = r.inv_ka = r.inv_kb = r.pa = r.pb = 0.0f;
+    r.scale = 0.0f;
+    int m;
+    const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
+    // reference quantities in x-voxel units (siddon.cu.cc:19-38)
+    const double xd = (D[0] - S[0]) / V.vs[0], yd = (D[1] - S[1]) / V.vs[0], zd = (D[2] - S[2]) / V.vs[0];
+    if (ref) {
+        if (adjoint && (flags & NEFCT_F_SHORT_SKIP) && sqrt(xd * xd + yd * yd) < 10.0) return;
+        const bool xdom = (adjoint && (flags & NEFCT_F_TIE_GT)) ? (fabs(xd) > fabs(yd)) : (fabs(xd) >= fabs(yd));
+        m = xdom ? 0 : 1;
+    } else {
+        const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+        m = (ax >= ay && ax >= az) ? 0 : ((ay >= az) ? 1 : 2);
+    }
+    if (d[m] == 0.0) return;
+    const int a = (m == 0) ? 1 : 0;
+    const int b = (m == 2) ? 1 : 2;
+    const double ka = d[a] / d[m], kb = d[b] / d[m];
+    const double Aa = uS[a] - ka * uS[m], Ab = uS[b] - kb * uS[m];
+    // The exact walk clips to the volume box.  The reference rule can credit an
+    // in-range voxel from a slab whose exact segment lies just outside the box
+    // (the Q2 attribution at a corner), so reference mode clips to the box
+    // widened by one cell on each transverse side; its per-voxel range tests
+    // then decide exactly as siddon.cu.cc:80-176 does.
+    const double mg = ref ? 1.0 : 0.0;
+    double tin = 0.0, tout = (double)V.n[m];
+    clip_axis(ka, Aa, -mg, V.n[a] + mg, tin, tout);
+    clip_axis(kb, Ab, -mg, V.n[b] + mg, tin, tout);
+    if (!(tout > tin)) return;
+    int s0 = (int)floor(tin), s1 = (int)ceil(tout);
+    s0 = max(s0, 0);
+    s1 = min(s1, V.n[m]);
+    if (s1 <= s0) return;
+    const int tc = (s0 + s1) >> 1;
+    const int ha = V.n[a] >> 1, hb = V.n[b] >> 1;
+    r.dom = m;
+    r.s0 = s0;
+    r.s1 = s1;
+    r.tc = tc;
+    r.ka = (float)ka;
+    r.kb = (float)kb;
+    r.Ac = (float)(Aa + ka * tc - ha);
+    r.Bc = (float)(Ab + kb * tc - hb);
+    r.inv_ka = (ka != 0.0) ? (float)(1.0 / ka) : 0.0f;
+    r.inv_kb = (kb != 0.0) ? (float)(1.0 / kb) : 0.0f;
+    r.pa = ka > 0.0 ? 1.0f : 0.0f;
+    r.pb = kb > 0.0 ? 1.0f : 0.0f;
+    const double qa = ka * V.vs[a] / V.vs[m], qb = kb * V.vs[b] / V.vs[m];
+    double sc = V.vs[m] * sqrt(1.0 + qa * qa + qb * qb);
+    if (flags & NEFCT_F_INV_L2) {
+        const double L2 = xd * xd + yd * yd + zd * zd;
+        sc = sc / L2;
+    }
+    r.scale = (float)sc;
+}
+
+__device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts& Ly)
+{
+    Walk w;
+    const int nx = V.n[0], ny = V.n[1], nz = V.n[2];
+    if (dom == 0) {            // x slabs: a = y (fast), b = z; y-fastest copy
+        w.sa = 1;
+        w.sb = Ly.NYP * nx;
+        w.st = Ly.NYP;
+        w.na = ny;
+        w.nb = nz;
+        w.ha = ny >> 1;
+        w.hb = nz >> 1;
+        w.org = (w.ha + PAD) + (w.hb + PAD) * w.sb;
+    } else if (dom == 1) {     // y slabs: a = x (fast), b = z; x-fastest copy
+        w.sa = 1;
+        w.sb = Ly.NXP * ny;
+        w.st = Ly.NXP;
+        w.na = nx;
+        w.nb = nz;
+        w.ha = nx >> 1;
+        w.hb = nz >> 1;
+        w.org = (w.ha + PAD) + (w.hb + PAD) * w.sb;
+    } else {                   // z slabs: a = x, b = y (unpadded: checked); x-fastest copy
+        w.sa = 1;
+        w.sb = Ly.NXP;
+        w.st = Ly.NXP * ny;
+        w.na = nx;
+        w.nb = ny;
+        w.ha = nx >> 1;
+        w.hb = ny >> 1;
+        w.org = (w.ha + PAD) + w.hb * w.sb + PAD * w.st;
+    }
+    return w;
+}
+
+// Per-slab segment decomposition.  Returns the three (centred) cells and
+// their fractions of the slab; the exact rule walks from the slab's lower
+// face, the reference rule (NEFCT_F_REF_SLAB) measures each axis from its
+// low-value end (siddon.cu.cc:52-76, 132-176).
+struct Seg {
+    int a1, b1, a2, b2, a3, b3;
+    float w1, w2, w3;
+    bool ok;     // reference outer range test (siddon.cu.cc:80,112,132)
+};
+
+template <bool REF>
+__device__ __forceinline__ void slab_segments(float a0, float a1, float fla0, float fla1, float b0, float b1,
+                                              float flb0, float flb1, const Ray& r, const Walk& w, Seg& s)
+{
+    const int ca0 = (int)fla0, ca1 = (int)fla1, cb0 = (int)flb0, cb1 = (int)flb1;
+    if (!REF) {
+        const float fa = (ca1 != ca0) ? clamp01((fla0 + r.pa - a0) * r.inv_ka) : 1.0f;
+        const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - b0) * r.inv_kb) : 1.0f;
+        const bool afirst = fa < fb;
+        const float f1 = fminf(fa, fb), f2 = fmaxf(fa, fb);
+        s.a1 = ca0; s.b1 = cb0;
+        s.a2 = afirst ? ca1 : ca0;
+        s.b2 = afirst ? cb0 : cb1;
+        s.a3 = ca1; s.b3 = cb1;
+        s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
+        s.ok = true;
+    } else {
+        const float loa = fminf(a0, a1), hia = fmaxf(a0, a1);
+        const float lob = fminf(b0, b1), hib = fmaxf(b0, b1);
+        const int cla = min(ca0, ca1), cha = max(ca0, ca1);
+        const int clb = min(cb0, cb1), chb = max(cb0, cb1);
+        const float ra = (cla != cha) ? ((float)cha - loa) / (hia - loa) : 1.0f;
+        const float rb = (clb != chb) ? ((float)chb - lob) / (hib - lob) : 1.0f;
+        const int tla = cla + w.ha, tlb = clb + w.hb;
+        const bool oka = (cla == cha) ? (tla >= 0 && tla < w.na) : (tla >= -1 && tla < w.na);
+        const bool okb = (clb == chb) ? (tlb >= 0 && tlb < w.nb) : (tlb >= -1 && tlb < w.nb);
+        const float f1 = fminf(ra, rb), f2 = fmaxf(ra, rb);
+        s.a1 = cla; s.b1 = clb;
+        s.a2 = (ra > rb) ? cla : cha;
+        s.b2 = (ra > rb) ? chb : clb;
+        s.a3 = cha; s.b3 = chb;
+        s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
+        s.ok = oka && okb;
+    }
+}
+
+__device__ __forceinline__ bool cell_in(const Walk& w, int ca, int cb)
+{
+    return (unsigned)(ca + w.ha) < (unsigned)w.na && (unsigned)(cb + w.hb) < (unsigned)w.nb;
+}
+
+// MODE 0: forward (returns the weighted sum), 1: adjoint scatter of 'val',
+// 2: count of voxels with nonzero length.
+template <int MODE, bool REF, bool CHECKED>
+__device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Walk& w,
+                          __amdgpu_buffer_rsrc_t rs, float val)
+{
+    float acc = 0.0f;
+    float tf = (float)(tmin - r.tc);
+    float a0 = fmaf(r.ka, tf, r.Ac), b0 = fmaf(r.kb, tf, r.Bc);
+    float fla0 = floorf(a0), flb0 = floorf(b0);
+    const float sv = val * r.scale;
+    for (int t = tmin; t < tmax; ++t) {
+        tf += 1.0f;
+        const float a1 = fmaf(r.ka, tf, r.Ac), b1 = fmaf(r.kb, tf, r.Bc);
+        const float fla1 = floorf(a1), flb1 = floorf(b1);
+        Seg s;
+        slab_segments<REF>(a0, a1, fla0, fla1, b0, b1, flb0, flb1, r, w, s);
+        const bool act = mine && (t >= r.s0) && (t < r.s1) && s.ok;
+        const int tb = t * w.st + w.org;
+        const int o1 = (s.a1 * w.sa + s.b1 * w.sb + tb) * 4;
+        const int o2 = (s.a2 * w.sa + s.b2 * w.sb + tb) * 4;
+        const int o3 = (s.a3 * w.sa + s.b3 * w.sb + tb) * 4;
+        bool u1 = act && s.w1 > 0.0f, u2 = act && s.w2 > 0.0f, u3 = act && s.w3 > 0.0f;
+        if (CHECKED || REF || MODE == 2) {
+            u1 = u1 && cell_in(w, s.a1, s.b1);
+            u2 = u2 && cell_in(w, s.a2, s.b2);
+            u3 = u3 && cell_in(w, s.a3, s.b3);
+        }
+        if (MODE == 0) {
+            float v = 0.0f;
+            if (u1) v = s.w1 * buf_load(rs, o1);
+            if (u2) v = fmaf(s.w2, buf_load(rs, o2), v);
+            if (u3) v = fmaf(s.w3, buf_load(rs, o3), v);
+            acc += v;
+        } else if (MODE == 1) {
+            if (u1) buf_atomic_add(rs, o1, s.w1 * sv);
+            if (u2) buf_atomic_add(rs, o2, s.w2 * sv);
+            if (u3) buf_atomic_add(rs, o3, s.w3 * sv);
+        } else {
+            acc += (float)((int)u1 + (int)u2 + (int)u3);
+        }
+        a0 = a1; b0 = b1; fla0 = fla1; flb0 = flb1;
+    }
+    return acc;
+}
+
+struct Launch {
+    VolDev V;
+    DetDev G;
+    Layouts Ly;
+    const double* trig;  // (cos, sin) per view
+    int nv;
+    unsigned flags;
+};
+
+template <int MODE, bool REF>
+__device__ float project_all_dominances(const Launch& P, const Ray& r, bool inside, float val)
+{
+    float acc = 0.0f;
+    // x slabs and y slabs in the padded copies; z slabs checked in the x copy
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (REF && d == 2) break;
+        const bool mine = inside && r.dom == d;
+        const int lo = __builtin_amdgcn_readfirstlane(wave_min_i32(mine ? r.s0 : 0x7fffffff));
+        const int hi = __builtin_amdgcn_readfirstlane(wave_max_i32(mine ? r.s1 : -0x7fffffff));
+        if (lo >= hi) continue;
+        const Walk w = walk_of(d, P.V, P.Ly);
+        float* base = (d == 0) ? P.Ly.yf : P.Ly.xf;
+        const uint32_t nbytes = (d == 0) ? P.Ly.yf_bytes : P.Ly.xf_bytes;
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, nbytes);
+        float v;
+        if (d == 2)
+            v = walk_ray<MODE, REF, true>(r, mine, lo, hi, w, rs, val);
+        else
+            v = walk_ray<MODE, REF, false>(r, mine, lo, hi, w, rs, val);
+        if (mine) acc = v;
+    }
+    return acc;
+}
+
+template <int MODE, bool REF>
+__global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __restrict__ pv_in,
+                                                     float* __restrict__ pv_out, int32_t* __restrict__ counts)
+{
+    const int ia = blockIdx.x * BX + threadIdx.x;
+    const int ib = blockIdx.y * BY + threadIdx.y;
+    const int iv = blockIdx.z;
+    const bool inside = ia < P.G.na && ib < P.G.nb;
+    const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
+    Ray r;
+    r.dom = -1;
+    r.s0 = r.s1 = r.tc = 0;
+    r.ka = r.kb = r.Ac = r.Bc = r.inv_ka = r.inv_kb = r.pa = r.pb = 0.0f;
+    r.scale = 0.0f;
+    if (inside) setup_ray(P.V, P.G, c, s, ia, ib, P.flags, MODE == 1, r);
+    const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
+    float val = 0.0f;
+    if (MODE == 1 && inside) val = pv_in[id];
+    const float acc = project_all_dominances<MODE, REF>(P, r, inside, val);
+    if (!inside) return;
+    if (MODE == 0) pv_out[id] = acc * r.scale;
+    if (MODE == 2) counts[id] = (int32_t)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint with LDS slab windows.
+//
+// A workgroup owns a tile of 64 columns x (4 * AR) rows of one view (each lane
+// walks AR rays).  All its rays step through the same slab t together.  The
+// segments of slab t are first summed into an LDS window covering the tile's
+// footprint in that slab; after one barrier the window is flushed with one
+// global float atomic per touched voxel (rows of the window are contiguous in
+// the padded layout, so the flush is coalesced).  Two windows alternate, so
+// one barrier per slab suffices.  A segment whose cell falls outside the
+// window (loose bound) goes straight to a global atomic, so the window bound
+// affects speed only, never the result.
+// ---------------------------------------------------------------------------
+constexpr int WIN_A = 128;
+constexpr int WIN_B = 32;
+constexpr int AR = 4;
+
+__device__ __forceinline__ float wave_min_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, NEFCT_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, NEFCT_WAVE));
+    return v;
+}
+
+template <bool REF, bool CHECKED>
+__device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[AR], const float (&sv)[AR], int lo,
+                                         int hi, const Walk& w, __amdgpu_buffer_rsrc_t rs, float (*win)[WIN_B][WIN_A],
+                                         const float (&bnd)[8], int wave, int lane)
+{
+    float a0[AR], b0[AR], fla0[AR], flb0[AR];
+#pragma unroll
+    for (int j = 0; j < AR; ++j) {
+        const float tf = (float)(lo - r[j].tc);
+        a0[j] = fmaf(r[j].ka, tf, r[j].Ac);
+        b0[j] = fmaf(r[j].kb, tf, r[j].Bc);
+        fla0[j] = floorf(a0[j]);
+        flb0[j] = floorf(b0[j]);
+    }
+    const float amn = bnd[0], amx = bnd[1], kamn = bnd[2], kamx = bnd[3];
+    const float bmn = bnd[4], bmx = bnd[5], kbmn = bnd[6], kbmx = bnd[7];
+    for (int t = lo; t < hi; ++t) {
+        const float dt0 = (float)(t - lo), dt1 = dt0 + 1.0f;
+        const float la_lo = amn + fminf(kamn * dt0, kamn * dt1), la_hi = amx + fmaxf(kamx * dt0, kamx * dt1);
+        const float lb_lo = bmn + fminf(kbmn * dt0, kbmn * dt1), lb_hi = bmx + fmaxf(kbmx * dt0, kbmx * dt1);
+        const int oa = __builtin_amdgcn_readfirstlane((int)floorf(la_lo) - 1);
+        const int ob = __builtin_amdgcn_readfirstlane((int)floorf(lb_lo) - 1);
+        const int span_a = __builtin_amdgcn_readfirstlane(min(max((int)floorf(la_hi) + 2 - oa, 0), WIN_A));
+        const int span_b = __builtin_amdgcn_readfirstlane(min(max((int)floorf(lb_hi) + 2 - ob, 0), WIN_B));
+        float(*cur)[WIN_A] = win[t & 1];
+        const int tb = t * w.st + w.org;
+#pragma unroll
+        for (int j = 0; j < AR; ++j) {
+            const float tf = (float)(t + 1 - r[j].tc);
+            const float a1 = fmaf(r[j].ka, tf, r[j].Ac), b1 = fmaf(r[j].kb, tf, r[j].Bc);
+            const float fla1 = floorf(a1), flb1 = floorf(b1);
+            Seg s;
+            slab_segments<REF>(a0[j], a1, fla0[j], fla1, b0[j], b1, flb0[j], flb1, r[j], w, s);
+            const bool act = mine[j] && (t >= r[j].s0) && (t < r[j].s1) && s.ok;
+            const int ca[3] = {s.a1, s.a2, s.a3};
+            const int cb[3] = {s.b1, s.b2, s.b3};
+            const float wk[3] = {s.w1, s.w2, s.w3};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                bool u = act && wk[k] > 0.0f;
+                if (CHECKED || REF) u = u && cell_in(w, ca[k], cb[k]);
+                if (u) {
+                    const float v = wk[k] * sv[j];
+                    const int la = ca[k] - oa, lb = cb[k] - ob;
+                    if ((unsigned)la < (unsigned)span_a && (unsigned)lb < (unsigned)span_b)
+                        atomicAdd(&cur[lb][la], v);
+                    else
+                        buf_atomic_add(rs, (ca[k] * w.sa + cb[k] * w.sb + tb) * 4, v);
+                }
+            }
+            a0[j] = a1;
+            b0[j] = b1;
+            fla0[j] = fla1;
+            flb0[j] = flb1;
+        }
+        __syncthreads();
+        for (int lb = wave; lb < span_b; lb += BY) {
+            for (int la = lane; la < span_a; la += BX) {
+                const float v = cur[lb][la];
+                if (v != 0.0f) {
+                    cur[lb][la] = 0.0f;
+                    const int ca = oa + la, cb = ob + lb;
+                    if (!(CHECKED || REF) || cell_in(w, ca, cb))
+                        buf_atomic_add(rs, (ca * w.sa + cb * w.sb + tb) * 4, v);
+                }
+            }
+        }
+    }
+}
+
+template <bool REF>
+__global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* __restrict__ pv_in)
+{
+    __shared__ float win[2][WIN_B][WIN_A];
+    __shared__ float redf[BY][8];
+    __shared__ int redi[BY][2];
+    const int lane = threadIdx.x, wave = threadIdx.y;
+    const int tid = lane + BX * wave;
+    {
+        float* wf = &win[0][0][0];
+        for (int i = tid; i < 2 * WIN_B * WIN_A; i += BX * BY) wf[i] = 0.0f;
+    }
+    const int ia = blockIdx.x * BX + lane;
+    const int iv = blockIdx.z;
+    const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
+    Ray r[AR];
+    bool in[AR];
+    float sv[AR];
+#pragma unroll
+    for (int j = 0; j < AR; ++j) {
+        const int ib = blockIdx.y * (BY * AR) + wave + BY * j;
+        in[j] = ia < P.G.na && ib < P.G.nb;
+        r[j].dom = -1;
+        r[j].s0 = r[j].s1 = r[j].tc = 0;
+        r[j].ka = r[j].kb = r[j].Ac = r[j].Bc = r[j].inv_ka = r[j].inv_kb = r[j].pa = r[j].pb = 0.0f;
+        r[j].scale = 0.0f;
+        sv[j] = 0.0f;
+        if (in[j]) {
+            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r[j]);
+            const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
+            sv[j] = pv_in[id] * r[j].scale;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (REF && d == 2) break;
+        bool mine[AR];
+        int lo = 0x7fffffff, hi = -0x7fffffff;
+        float bnd[8] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+#pragma unroll
+        for (int j = 0; j < AR; ++j) {
+            mine[j] = in[j] && r[j].dom == d;
+            if (mine[j]) {
+                lo = min(lo, r[j].s0);
+                hi = max(hi, r[j].s1);
+            }
+        }
+        lo = wave_min_i32(lo);
+        hi = wave_max_i32(hi);
+        if (lane == 0) {
+            redi[wave][0] = lo;
+            redi[wave][1] = hi;
+        }
+        __syncthreads();
+        lo = min(min(redi[0][0], redi[1][0]), min(redi[2][0], redi[3][0]));
+        hi = max(max(redi[0][1], redi[1][1]), max(redi[2][1], redi[3][1]));
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        hi = __builtin_amdgcn_readfirstlane(hi);
+        __syncthreads();
+        if (lo >= hi) continue;
+#pragma unroll
+        for (int j = 0; j < AR; ++j) {
+            if (mine[j]) {
+                const float tf = (float)(lo - r[j].tc);
+                const float a = fmaf(r[j].ka, tf, r[j].Ac), b = fmaf(r[j].kb, tf, r[j].Bc);
+                bnd[0] = fminf(bnd[0], a);
+                bnd[1] = fmaxf(bnd[1], a);
+                bnd[2] = fminf(bnd[2], r[j].ka);
+                bnd[3] = fmaxf(bnd[3], r[j].ka);
+                bnd[4] = fminf(bnd[4], b);
+                bnd[5] = fmaxf(bnd[5], b);
+                bnd[6] = fminf(bnd[6], r[j].kb);
+                bnd[7] = fmaxf(bnd[7], r[j].kb);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            bnd[q] = wave_min_f(bnd[q]);
+            bnd[q + 1] = wave_max_f(bnd[q + 1]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) redf[wave][q] = bnd[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            float mn = redf[0][q], mx = redf[0][q + 1];
+            for (int wv = 1; wv < BY; ++wv) {
+                mn = fminf(mn, redf[wv][q]);
+                mx = fmaxf(mx, redf[wv][q + 1]);
+            }
+            bnd[q] = mn;
+            bnd[q + 1] = mx;
+        }
+        __syncthreads();
+        // a bound that is not finite (no active ray had finite data) disables the window
+        if (!(fabsf(bnd[0]) < 1e6f && fabsf(bnd[1]) < 1e6f && fabsf(bnd[4]) < 1e6f && fabsf(bnd[5]) < 1e6f)) {
+            bnd[0] = bnd[1] = bnd[4] = bnd[5] = 0.0f;
+            bnd[2] = bnd[3] = bnd[6] = bnd[7] = 0.0f;
+        }
+        const Walk w = walk_of(d, P.V, P.Ly);
+        float* base = (d == 0) ? P.Ly.yf : P.Ly.xf;
+        const uint32_t nbytes = (d == 0) ? P.Ly.yf_bytes : P.Ly.xf_bytes;
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, nbytes);
+        if (d == 2)
+            adj_walk<REF, true>(r, mine, sv, lo, hi, w, rs, win, bnd, wave, lane);
+        else
+            adj_walk<REF, false>(r, mine, sv, lo, hi, w, rs, win, bnd, wave, lane);
+        __syncthreads();
+    }
+}
+
+__global__ void k_view_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const double a = (double)angles[i];
+    trig[2 * i] = cos(a);
+    trig[2 * i + 1] = sin(a);
+}
+
+// Fill [p, p+n) with zeros (a kernel, not hipMemsetAsync, so it records
+// cleanly into HIP graphs).
+__global__ void k_zero(float* __restrict__ p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+// x-fastest padded copy: out[(px) + NXP*(iy + ny*pz)]
+__global__ void k_pad_x(const float* __restrict__ img, int nx, int ny, int nz, int NXP, float* __restrict__ out)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int pz = blockIdx.z;
+    if (px >= NXP) return;
+    const int ix = px - PAD, iz = pz - PAD;
+    float v = 0.0f;
+    if (ix >= 0 && ix < nx && iz >= 0 && iz < nz) v = img[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)];
+    out[(size_t)px + (size_t)NXP * ((size_t)iy + (size_t)ny * pz)] = v;
+}
+
+// y-fastest padded copy through a 64x64 LDS tile: out[(py) + NYP*(ix + nx*pz)]
+__global__ __launch_bounds__(256) void k_pad_y(const float* __restrict__ img, int nx, int ny, int nz, int NYP,
+                                               float* __restrict__ out)
+{
+    __shared__ float tile[64][65];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64, pz = blockIdx.z;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int iz = pz - PAD;
+    const bool zin = iz >= 0 && iz < nz;
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+        const int iy = y0 + r - PAD, ix = x0 + tx;
+        float v = 0.0f;
+        if (zin && iy >= 0 && iy < ny && ix < nx) v = img[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)];
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+        const int ix = x0 + r, py = y0 + tx;
+        if (ix < nx && py < NYP) out[(size_t)py + (size_t)NYP * ((size_t)ix + (size_t)nx * pz)] = tile[tx][r];
+    }
+}
+
+// image[ix + nx*(iy + ny*iz)] = XF + YF accumulators (adjoint epilogue), tiled.
+__global__ __launch_bounds__(256) void k_combine(const float* __restrict__ xf, const float* __restrict__ yf,
+                                                 int nx, int ny, int nz, int NXP, int NYP,
+                                                 float* __restrict__ img)
+{
+    __shared__ float tile[64][65];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64, iz = blockIdx.z;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int pz = iz + PAD;
+    // load y-fastest tile: rows ix, columns iy
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+        const int ix = x0 + r, iy = y0 + tx;
+        float v = 0.0f;
+        if (ix < nx && iy < ny) v = yf[(size_t)(iy + PAD) + (size_t)NYP * ((size_t)ix + (size_t)nx * pz)];
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = ty; r < 64; r += 4) {
+        const int iy = y0 + r, ix = x0 + tx;
+        if (ix < nx && iy < ny) {
+            const float a = xf[(size_t)(ix + PAD) + (size_t)NXP * ((size_t)iy + (size_t)ny * pz)];
+            img[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)] = a + tile[tx][r];
+        }
+    }
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct WsPlan {
+    size_t trig_off, xf_off, yf_off, total;
+    size_t xf_elems, yf_elems;
+    int NXP, NYP, NZP;
+};
+
+WsPlan plan_ws(const int grid[3], int nv)
+{
+    WsPlan p;
+    const size_t nx = grid[0], ny = grid[1], nz = grid[2];
+    p.NXP = (int)nx + 2 * PAD;
+    p.NYP = (int)ny + 2 * PAD;
+    p.NZP = (int)nz + 2 * PAD;
+    p.xf_elems = (size_t)p.NXP * ny * (size_t)p.NZP;
+    p.yf_elems = (size_t)p.NYP * nx * (size_t)p.NZP;
+    p.trig_off = 0;
+    p.xf_off = align256(sizeof(double) * 2 * (size_t)(nv > 0 ? nv : 1));
+    p.yf_off = p.xf_off + align256(sizeof(float) * p.xf_elems);
+    p.total = p.yf_off + align256(sizeof(float) * p.yf_elems);
+    return p;
+}
+
+bool finite_all(std::initializer_list<double> v)
+{
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+int validate(const int grid[3], const float center[3], const float size[3], const float* angles, int nv,
+             float SID, float SAD, int na, float da, int nb, float db, float ai, float bi, unsigned flags)
+{
+    if (!grid || !center || !size || !angles) return NEFCT_ERR_NULL;
+    if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
+    if (nb > 65535 * BY || nv > 65535) return NEFCT_ERR_SHAPE;
+    if (grid[0] > 16384 || grid[1] > 16384 || grid[2] > 16384) return NEFCT_ERR_SHAPE;
+    const int grid_arr[3] = {grid[0], grid[1], grid[2]};
+    const WsPlan p = plan_ws(grid_arr, nv);
+    if (p.xf_elems * 4 >= 0x7fffffffull || p.yf_elems * 4 >= 0x7fffffffull) return NEFCT_ERR_SHAPE;
+    if (!finite_all({center[0], center[1], center[2], size[0], size[1], size[2], SID, SAD, da, db, ai, bi}))
+        return NEFCT_ERR_GEOMETRY;
+    if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
+    if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
+    if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT))
+        return NEFCT_ERR_UNSUPPORTED;
+    if (flags & NEFCT_F_REF_SLAB) {
+        // the reference y-walk treats y slabs as x-voxel thick (siddon.cu.cc:188-201)
+        const double vx = (double)size[0] / grid[0], vy = (double)size[1] / grid[1];
+        if (fabs(vx - vy) > 1e-6 * vx) return NEFCT_ERR_UNSUPPORTED;
+    }
+    return NEFCT_OK;
+}
+
+Launch make_launch(const int grid[3], const float center[3], const float size[3], int nv, float SID, float SAD,
+                   int na, float da, float ai, int nb, float db, float bi, int cyl, unsigned flags, char* ws,
+                   const WsPlan& p)
+{
+    Launch L;
+    for (int i = 0; i < 3; ++i) {
+        L.V.n[i] = grid[i];
+        L.V.vs[i] = (double)size[i] / grid[i];
+        L.V.lo[i] = (double)center[i] - (double)size[i] * 0.5;
+    }
+    L.G.cyl = cyl;
+    L.G.sid = SID;
+    L.G.sad = SAD;
+    L.G.na = na;
+    L.G.nb = nb;
+    L.G.da = da;
+    L.G.ai = ai;
+    L.G.db = db;
+    L.G.bi = bi;
+    L.Ly.xf = (float*)(ws + p.xf_off);
+    L.Ly.yf = (float*)(ws + p.yf_off);
+    L.Ly.xf_bytes = (uint32_t)(p.xf_elems * 4);
+    L.Ly.yf_bytes = (uint32_t)(p.yf_elems * 4);
+    L.Ly.NXP = p.NXP;
+    L.Ly.NYP = p.NYP;
+    L.Ly.NZP = p.NZP;
+    L.trig = (const double*)(ws + p.trig_off);
+    L.nv = nv;
+    L.flags = flags;
+    return L;
+}
+
+void launch_trig(const float* angles, int nv, double* trig, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_view_trig, dim3((nv + 255) / 256), dim3(256), 0, st, angles, nv, trig);
+}
+
+void launch_pad(const float* image, const int grid[3], const WsPlan& p, char* ws, hipStream_t st)
+{
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    float* xf = (float*)(ws + p.xf_off);
+    float* yf = (float*)(ws + p.yf_off);
+    hipLaunchKernelGGL(k_pad_x, dim3((p.NXP + 255) / 256, ny, p.NZP), dim3(256), 0, st, image, nx, ny, nz, p.NXP, xf);
+    hipLaunchKernelGGL(k_pad_y, dim3((nx + 63) / 64, (p.NYP + 63) / 64, p.NZP), dim3(256), 0, st, image, nx, ny, nz,
+                       p.NYP, yf);
+}
+
+void launch_zero(float* ptr, size_t n, hipStream_t st)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, ptr, n);
+}
+
+int run_forward(const float* image, const int grid[3], const float center[3], const float size[3],
+                const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb, float db,
+                float bi, int cyl, float* out, unsigned flags, void* workspace, size_t wsbytes, void* stream)
+{
+    int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
+    if (st) return st;
+    if (!image || !out) return NEFCT_ERR_NULL;
+    const WsPlan p = plan_ws(grid, nv);
+    if (!workspace || wsbytes < p.total) return NEFCT_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
+    launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
+    launch_pad(image, grid, p, ws, s);
+    const dim3 g((na + BX - 1) / BX, (nb + BY - 1) / BY, nv), b(BX, BY);
+    if (flags & NEFCT_F_REF_SLAB)
+        hipLaunchKernelGGL((k_siddon<0, true>), g, b, 0, s, L, (const float*)nullptr, out, (int32_t*)nullptr);
+    else
+        hipLaunchKernelGGL((k_siddon<0, false>), g, b, 0, s, L, (const float*)nullptr, out, (int32_t*)nullptr);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+int run_adjoint(const float* pv, const int grid[3], const float center[3], const float size[3], const float* angles,
+                int nv, float SID, float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
+                float* image, unsigned flags, void* workspace, size_t wsbytes, void* stream)
+{
+    int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
+    if (st) return st;
+    if (!image || !pv) return NEFCT_ERR_NULL;
+    const WsPlan p = plan_ws(grid, nv);
+    if (!workspace || wsbytes < p.total) return NEFCT_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
+    launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
+    launch_zero((float*)(ws + p.xf_off), p.xf_elems, s);
+    launch_zero((float*)(ws + p.yf_off), p.yf_elems, s);
+    const dim3 g((na + BX - 1) / BX, (nb + BY * AR - 1) / (BY * AR), nv), b(BX, BY);
+    if (flags & NEFCT_F_REF_SLAB)
+        hipLaunchKernelGGL((k_siddon_adj<true>), g, b, 0, s, L, pv);
+    else
+        hipLaunchKernelGGL((k_siddon_adj<false>), g, b, 0, s, L, pv);
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    hipLaunchKernelGGL(k_combine, dim3((nx + 63) / 64, (ny + 63) / 64, nz), dim3(256), 0, s,
+                       (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP, image);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nefct_status_string(int status)
+{
+    switch (status) {
+    case NEFCT_OK: return "ok";
+    case NEFCT_ERR_NULL: return "null pointer argument";
+    case NEFCT_ERR_SHAPE: return "invalid or oversized shape";
+    case NEFCT_ERR_GEOMETRY: return "non-finite or degenerate geometry";
+    case NEFCT_ERR_WORKSPACE: return "workspace missing or too small";
+    case NEFCT_ERR_UNSUPPORTED: return "unsupported flags or configuration";
+    case NEFCT_ERR_LAUNCH: return "HIP kernel launch failed";
+    default: return "unknown status";
+    }
+}
+
+size_t nefct_siddon_workspace_size(const int grid[3], int nv)
+{
+    if (!grid || grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0) return 0;
+    return plan_ws(grid, nv).total;
+}
+
+int nefct_project_flat(const float* image, const int grid[3], const float center[3], const float size[3],
+                       const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb,
+                       float db, float bi, float* projection_value, unsigned flags, void* workspace,
+                       size_t workspace_bytes, void* stream)
+{
+    return run_forward(image, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 0,
+                       projection_value, flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_project_cyli(const float* image, const int grid[3], const float center[3], const float size[3],
+                       const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb,
+                       float db, float bi, float* projection_value, unsigned flags, void* workspace,
+                       size_t workspace_bytes, void* stream)
+{
+    return run_forward(image, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 1,
+                       projection_value, flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_backproject_flat(const float* projection_value, const int grid[3], const float center[3],
+                           const float size[3], const float* angles, int nv, float SID, float SAD, int na, float da,
+                           float ai, int nb, float db, float bi, float* image, unsigned flags, void* workspace,
+                           size_t workspace_bytes, void* stream)
+{
+    return run_adjoint(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 0, image,
+                       flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_backproject_cyli(const float* projection_value, const int grid[3], const float center[3],
+                           const float size[3], const float* angles, int nv, float SID, float SAD, int na, float da,
+                           float ai, int nb, float db, float bi, float* image, unsigned flags, void* workspace,
+                           size_t workspace_bytes, void* stream)
+{
+    return run_adjoint(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 1, image,
+                       flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_siddon_count(const int grid[3], const float center[3], const float size[3], const float* angles, int nv,
+                       float SID, float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
+                       int32_t* counts, unsigned flags, void* workspace, size_t workspace_bytes, void* stream)
+{
+    int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
+    if (st) return st;
+    if (!counts) return NEFCT_ERR_NULL;
+    const WsPlan p = plan_ws(grid, nv);
+    if (!workspace || workspace_bytes < p.total) return NEFCT_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
+    launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
+    const dim3 g((na + BX - 1) / BX, (nb + BY - 1) / BY, nv), b(BX, BY);
+    if (flags & NEFCT_F_REF_SLAB)
+        hipLaunchKernelGGL((k_siddon<2, true>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
+    else
+        hipLaunchKernelGGL((k_siddon<2, false>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+"""ctypes binding of libnefct_hip.so (the C ABI declared in include/nefct_hip.h).
+
+The library is built in-tree by ``ct-nef_amd/build.py``.  There is no CPU
+fallback: if the library is missing or no HIP device is present, every
+operator raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnefct_hip.so")
+
+# flags (include/nefct_hip.h)
+F_INV_L2 = 1
+F_REF_SLAB = 2
+F_SHORT_SKIP = 4
+F_TIE_GT = 8
+REF_FORWARD = F_INV_L2 | F_REF_SLAB
+REF_ADJOINT = F_REF_SLAB | F_SHORT_SKIP | F_TIE_GT
+
+_c_int3 = ctypes.c_int * 3
+_c_float3 = ctypes.c_float * 3
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_u = ctypes.c_uint
+_sz = ctypes.c_size_t
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "nefct_status_string": [_i],
+    "nefct_siddon_workspace_size": [_vp, _i],
+    "nefct_project_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_project_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_backproject_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_siddon_count": [_vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u, _vp, _sz, _vp],
+}
+_RESTYPE = {
+    "nefct_status_string": ctypes.c_char_p,
+    "nefct_siddon_workspace_size": ctypes.c_size_t,
+}
+
+_lib = None
+
+
+class NefctError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NefctError(f"{LIB_PATH} is missing: run ct-nef_amd/build.py (hipcc, gfx950) first")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, ctypes.c_int)
+        _lib = lib
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = load().nefct_status_string(int(status)).decode()
+        raise NefctError(f"{what}: {msg} (status {status})")
+
+
+def int3(v):
+    return _c_int3(*(int(x) for x in v))
+
+
+def float3(v):
+    return _c_float3(*(float(x) for x in v))

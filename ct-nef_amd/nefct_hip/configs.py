@@ -1,0 +1,60 @@
+"""The five benchmark workloads as concrete geometries.
+
+C1 and C2 use source-to-isocentre 1000 mm and source-to-detector 1500 mm (the
+values given for C1; at magnification 1.5 C2's 1024 x 0.6 mm detector covers
+exactly the 409.6 mm volume width).  C3-C5 do not fix the distances: they use
+SDD = 1.5 SOD and a detector pitch = voxel * 1.5 * side / cells, so the
+detector covers the volume.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+import math
+
+import numpy as np
+
+from .geometry import ConeGeometry, VolumeGeometry
+
+
+@dataclass(frozen=True)
+class Workload:
+    name: str
+    vol: VolumeGeometry
+    det: ConeGeometry
+    nv: int
+    ellipsoids: int
+    seed: int
+
+    def angles(self):
+        return (2 * math.pi * np.arange(self.nv) / self.nv).astype(np.float32)
+
+    @property
+    def rays(self):
+        return self.nv * self.det.na * self.det.nb
+
+
+def c1():
+    """128^3 @ 1.0 mm, 256 x 256 @ 1.5 mm, 180 views over 360 deg, 10 ellipsoids."""
+    return Workload("C1 128^3 / 256x256 / 180 views", VolumeGeometry.cube((128, 128, 128), 1.0),
+                    ConeGeometry.flat(1000.0, 1500.0, 256, 1.5, 256, 1.5), 180, 10, 1)
+
+
+def c2():
+    """512x512x256 @ 0.8 mm, 1024 x 768 @ 0.6 mm, 720 views over 360 deg, 30 ellipsoids."""
+    return Workload("C2 512x512x256 / 1024x768 / 720 views", VolumeGeometry.cube((512, 512, 256), 0.8),
+                    ConeGeometry.flat(1000.0, 1500.0, 1024, 0.6, 768, 0.6), 720, 30, 2)
+
+
+def c3():
+    """256^3 @ 1.0 mm, 512 x 384, 360 views (SOD 1000, SDD 1500, pitch covering the volume)."""
+    return Workload("C3 256^3 / 512x384 / 360 views", VolumeGeometry.cube((256, 256, 256), 1.0),
+                    ConeGeometry.flat(1000.0, 1500.0, 512, 0.75, 384, 1.0), 360, 10, 3)
+
+
+def short_scan_views(wl: Workload):
+    """Indices of the views covering 180 deg + fan angle, starting at view 0."""
+    fan = 2 * wl.det.fan_half_angle()
+    span = math.pi + fan
+    step = 2 * math.pi / wl.nv
+    n = int(math.ceil(span / step)) + 1
+    return np.arange(min(n, wl.nv))

@@ -1,0 +1,88 @@
+"""Seeded synthetic inputs: ellipsoid phantoms and smooth displacement fields.
+
+The ellipsoid rasteriser follows the 3-D branch of the reference's Shepp-Logan
+generator (nefct/data/sl_phantom.py:109-148): indicator of a rotated
+ellipsoid (ZXZ Euler angles, :129-145) sampled at voxel centres, intensities
+added.  Parameters are drawn from ``numpy.random.Generator(PCG64(seed))``:
+semi-axes U[5, 50] mm, attenuation U[0.01, 0.03] /mm, centres U(+-0.35 size)
+per axis, Euler angles U[0, 2 pi).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .geometry import VolumeGeometry
+
+
+def ellipsoid_params(seed: int, count: int, size_mm):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        axes = rng.uniform(5.0, 50.0, 3)
+        mu = rng.uniform(0.01, 0.03)
+        centre = np.array([rng.uniform(-0.35 * s, 0.35 * s) for s in size_mm])
+        euler = rng.uniform(0.0, 2 * math.pi, 3)
+        out.append((axes, mu, centre, euler))
+    return out
+
+
+def _zxz(phi, theta, psi):
+    cphi, sphi = math.cos(phi), math.sin(phi)
+    cth, sth = math.cos(theta), math.sin(theta)
+    cpsi, spsi = math.cos(psi), math.sin(psi)
+    # rows map world coordinates to the ellipsoid frame (sl_phantom.py:137-143)
+    return np.array([[cpsi * cphi - cth * sphi * spsi, cpsi * sphi + cth * cphi * spsi, spsi * sth],
+                     [-spsi * cphi - cth * sphi * cpsi, -spsi * sphi + cth * cphi * cpsi, cpsi * sth],
+                     [sth * sphi, -sth * cphi, cth]])
+
+
+def ellipsoid_phantom(vol: VolumeGeometry, seed: int, count: int, device="cpu", dtype=torch.float32):
+    """Volume [nz, ny, nx] of additive ellipsoids (attenuation in 1/mm)."""
+    nx, ny, nz = vol.grid
+    vx, vy, vz = vol.voxel
+    img = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    x = torch.tensor(vol.center[0] - vol.size[0] / 2 + (np.arange(nx) + 0.5) * vx, dtype=torch.float64, device=device)
+    y = torch.tensor(vol.center[1] - vol.size[1] / 2 + (np.arange(ny) + 0.5) * vy, dtype=torch.float64, device=device)
+    z = torch.tensor(vol.center[2] - vol.size[2] / 2 + (np.arange(nz) + 0.5) * vz, dtype=torch.float64, device=device)
+    for axes, mu, centre, euler in ellipsoid_params(seed, count, vol.size):
+        R = _zxz(*euler)
+        r = float(axes.max())
+        # bounding box in index space
+        lo = [int(max(0, math.floor((centre[i] + vol.center[i] - r - (vol.center[i] - vol.size[i] / 2)) / vs)))
+              for i, vs in enumerate((vx, vy, vz))]
+        hi = [int(min(n, math.ceil((centre[i] + vol.center[i] + r - (vol.center[i] - vol.size[i] / 2)) / vs) + 1))
+              for i, (n, vs) in enumerate(zip((nx, ny, nz), (vx, vy, vz)))]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        X = x[lo[0]:hi[0]] - (centre[0] + vol.center[0])
+        Y = y[lo[1]:hi[1]] - (centre[1] + vol.center[1])
+        Z = z[lo[2]:hi[2]] - (centre[2] + vol.center[2])
+        Zg, Yg, Xg = torch.meshgrid(Z, Y, X, indexing="ij")
+        u = Xg * R[0, 0] + Yg * R[0, 1] + Zg * R[0, 2]
+        v = Xg * R[1, 0] + Yg * R[1, 1] + Zg * R[1, 2]
+        w = Xg * R[2, 0] + Yg * R[2, 1] + Zg * R[2, 2]
+        inside = (u / axes[0]) ** 2 + (v / axes[1]) ** 2 + (w / axes[2]) ** 2 <= 1.0
+        img[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += inside.to(torch.float64) * mu
+    return img.to(dtype)
+
+
+def gaussian_dvf(shape_zyx, seed: int, voxel_mm=1.0, bumps=8, device="cpu"):
+    """Smooth displacement field [3, nz, ny, nx] (mm, z component only): sum of
+    Gaussian bumps, sigma U[20, 40] voxels, peak U[0, 10] mm, centres uniform."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nz, ny, nx = shape_zyx
+    iz = torch.arange(nz, dtype=torch.float64, device=device)[:, None, None]
+    iy = torch.arange(ny, dtype=torch.float64, device=device)[None, :, None]
+    ix = torch.arange(nx, dtype=torch.float64, device=device)[None, None, :]
+    dz = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    for _ in range(bumps):
+        sigma = rng.uniform(20.0, 40.0)
+        peak = rng.uniform(0.0, 10.0)
+        c = (rng.uniform(0, nx), rng.uniform(0, ny), rng.uniform(0, nz))
+        dz += peak * torch.exp(-((ix - c[0]) ** 2 + (iy - c[1]) ** 2 + (iz - c[2]) ** 2) / (2 * sigma * sigma))
+    dvf = torch.zeros((3, nz, ny, nx), dtype=torch.float32, device=device)
+    dvf[2] = dz.to(torch.float32)
+    return dvf

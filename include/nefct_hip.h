@@ -1,0 +1,112 @@
+/*
+ * nefct_hip.h -- C ABI of the MI355X (gfx950) cone-beam projector library.
+ *
+ * Drop-in boundary for the TensorFlow custom ops of bill52547/CT-NEF:
+ *   ProjectFlat / ProjectCyli / BackProjectFlat / BackProjectCyli
+ *     (nefct/cc/siddon_3d/siddon.cc:11-80, launchers siddon.cu.cc:392-448,834-890)
+ *   Deform (cpp/deform/deform.cc:12-22, deform.cu.cc:112-126)
+ *   FDK weighted backprojection (nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:257-298,
+ *     weights/filter of mat_codes/filtering.m and ParkerWeight.m)
+ *
+ * Conventions (all entry points):
+ *   - every array argument is a device pointer owned by the caller, except
+ *     grid/center/size, which are HOST arrays of 3 (the reference copies them
+ *     device->host synchronously on every call, siddon.cu.cc:403-405);
+ *   - volumes are x fastest: image[ix + nx*(iy + ny*iz)];
+ *     sinograms are pv[ia + na*(ib + nb*iv)] (shape {nv, nb, na});
+ *   - geometry uses the reference siddon-op names: SID = source-to-isocentre,
+ *     SAD = source-to-detector (siddon.cu.cc:344-348);
+ *     a = (ai + ia) da - na da / 2, b = (bi + ib) db - nb db / 2;
+ *   - 'stream' is a hipStream_t passed as void* (NULL = legacy default stream);
+ *     no entry point synchronises the host;
+ *   - outputs are fully written (zero-filled where the reference zero-fills);
+ *   - return 0 (NEFCT_OK) or a negative nefct_status; nothing is launched on
+ *     error.  Calls that overlap on one stream-free workspace race.
+ */
+#ifndef NEFCT_HIP_H
+#define NEFCT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NEFCT_OK = 0,
+    NEFCT_ERR_NULL = -1,        /* a required pointer is NULL */
+    NEFCT_ERR_SHAPE = -2,       /* non-positive or oversized dimension */
+    NEFCT_ERR_GEOMETRY = -3,    /* non-finite or degenerate geometry */
+    NEFCT_ERR_WORKSPACE = -4,   /* workspace missing or too small */
+    NEFCT_ERR_UNSUPPORTED = -5, /* flag combination not supported */
+    NEFCT_ERR_LAUNCH = -6       /* HIP launch failed */
+} nefct_status;
+
+/* Flags of the Siddon entry points.  0 = exact Siddon (the adjoint pair).
+ * The reference op's behaviour is reproduced by:
+ *   forward  NEFCT_REF_FORWARD  = INV_L2 | REF_SLAB
+ *   adjoint  NEFCT_REF_ADJOINT  = REF_SLAB | SHORT_SKIP | TIE_GT          */
+#define NEFCT_F_INV_L2 1u     /* Q1: weights / L^2, L in x-voxel units (siddon.cu.cc:86) */
+#define NEFCT_F_REF_SLAB 2u   /* Q2: reference slab walk and attribution (siddon.cu.cc:41-176) */
+#define NEFCT_F_SHORT_SKIP 4u /* Q3: adjoint skips rays with in-plane length < 10 voxels (:472) */
+#define NEFCT_F_TIE_GT 8u     /* Q6: adjoint x-walk only if |xd| > |yd| (:480) */
+#define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
+#define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
+
+/* Message for a status code (static string). */
+const char* nefct_status_string(int status);
+
+/* Bytes of device workspace the Siddon entry points need for this volume and
+ * view count (padded volume copies + per-view table).  Replaces the
+ * reference's implicit TF allocations; no reference counterpart. */
+size_t nefct_siddon_workspace_size(const int grid[3], int nv);
+
+/* Forward projection, flat detector.
+ * Replaces ProjectFlatOp::Compute (siddon.cc:129-157) + project_flat
+ * (siddon.cu.cc:392-419).  Output projection_value {nv, nb, na}, fully written.
+ * NOTE: the reference op passes da for db (siddon.cc:155); this entry point
+ * uses db as given (the Python drop-in reproduces the reference call). */
+int nefct_project_flat(const float* image, const int grid[3], const float center[3], const float size[3],
+                       const float* angles, int nv, float SID, float SAD, int na, float da, float ai,
+                       int nb, float db, float bi, float* projection_value, unsigned flags,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Forward projection, cylindrical detector (a in radians).
+ * Replaces ProjectCyliOp::Compute (siddon.cc:185-213) + project_cyli (siddon.cu.cc:421-448). */
+int nefct_project_cyli(const float* image, const int grid[3], const float center[3], const float size[3],
+                       const float* angles, int nv, float SID, float SAD, int na, float da, float ai,
+                       int nb, float db, float bi, float* projection_value, unsigned flags,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Adjoint (ray-driven atomic scatter), flat detector.  Output image
+ * x fastest, zero-filled then accumulated.
+ * Replaces BackProjectFlatOp::Compute (siddon.cc:244-272) + backproject_flat
+ * (siddon.cu.cc:834-861). */
+int nefct_backproject_flat(const float* projection_value, const int grid[3], const float center[3],
+                           const float size[3], const float* angles, int nv, float SID, float SAD,
+                           int na, float da, float ai, int nb, float db, float bi, float* image,
+                           unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Adjoint, cylindrical detector.  Replaces BackProjectCyliOp::Compute
+ * (siddon.cc:306-334) + backproject_cyli (siddon.cu.cc:863-890); reads the
+ * sinogram with the flat indexing (the reference's transposed index, Q5 at
+ * siddon.cu.cc:830, is not reproduced). */
+int nefct_backproject_cyli(const float* projection_value, const int grid[3], const float center[3],
+                           const float size[3], const float* angles, int nv, float SID, float SAD,
+                           int na, float da, float ai, int nb, float db, float bi, float* image,
+                           unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Per-ray count of voxels with nonzero length (N_int) as walked by the exact
+ * kernels; counts {nv, nb, na} int32.  No reference counterpart: it is the
+ * numerator of the bandwidth figures (voxel reads per ray). */
+int nefct_siddon_count(const int grid[3], const float center[3], const float size[3],
+                       const float* angles, int nv, float SID, float SAD, int na, float da, float ai,
+                       int nb, float db, float bi, int cyl, int32_t* counts, unsigned flags,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEFCT_HIP_H */

@@ -1,0 +1,222 @@
+"""GPU parity of the Siddon pair against the CPU oracle (HIP path through the C ABI).
+
+Bars: exact forward within 1e-5 relative L2 of the FP64 oracle; reference-mode
+forward/adjoint within 1e-5 of the FP32 restatement of
+nefct/cc/siddon_3d/siddon.cu.cc; exact adjoint within 1e-5 of the FP64
+transpose; GPU dot test <Ax, y> = <x, A^T y> to 1e-6 with FP64 accumulation.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _nh():
+    import nefct_hip
+    nefct_hip.load()
+    return nefct_hip
+
+
+def _relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _case(oracle, vol, det, nv, seed=0, views=None, k=8):
+    nh = _nh()
+    from nefct_hip.phantom import ellipsoid_phantom
+    ang = (2 * math.pi * np.arange(nv) / nv).astype(np.float32) + np.float32(0.013)
+    if views is not None:
+        ang = ang[views]
+    img = ellipsoid_phantom(vol, seed, k)
+    ov = oracle.vol_struct(vol.grid, vol.center, vol.size)
+    od = oracle.det_struct(det.cyl, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi)
+    return nh, ang, img, ov, od
+
+
+def _geoms():
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    return [
+        ("c1-like", VolumeGeometry.cube((64, 64, 64), 2.0), ConeGeometry.flat(1000.0, 1500.0, 128, 3.0, 128, 3.0)),
+        ("aniso-offset", VolumeGeometry((48, 40, 36), (3.0, -2.0, 1.5), (48 * 1.1, 40 * 1.1, 36 * 0.9)),
+         ConeGeometry.flat(300.0, 500.0, 70, 2.3, 45, 2.1, ai=0.3, bi=0.6)),
+        ("cyli", VolumeGeometry.cube((48, 48, 40), 1.5),
+         ConeGeometry("cyli", 400.0, 700.0, 90, 0.0035, 0.5, 50, 1.9, 0.5)),
+    ]
+
+
+@pytest.mark.parametrize("gi", [0, 1, 2])
+def test_forward_exact_vs_fp64_oracle(oracle, gi):
+    name, vol, det = _geoms()[gi]
+    nh, ang, img, ov, od = _case(oracle, vol, det, 36, seed=gi + 1)
+    got = nh.project(img.to(DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_fwd(ov, od, ang.astype(np.float64), img.double().numpy())
+    assert got.shape == ref.shape
+    assert _relerr(got, ref) < 1e-5, (name, _relerr(got, ref))
+
+
+@pytest.mark.parametrize("gi,nv,seed", [(0, 36, 7), (2, 36, 9), (0, 24, 3)])
+def test_forward_reference_mode_vs_restatement(oracle, gi, nv, seed):
+    """(0, 24, 3) holds rays that miss the volume box by a hair at its bottom
+    corners, which the reference's slab attribution still credits to an
+    in-range voxel (view 3 row 30 column 88 and three more)."""
+    name, vol, det = _geoms()[gi]
+    nh, ang, img, ov, od = _case(oracle, vol, det, nv, seed=seed)
+    got = nh.project(img.to(DEV), vol, det, torch.tensor(ang), flags=nh.REF_FORWARD).cpu().numpy()
+    ref = oracle.ref_fwd(ov, od, ang, img.numpy(), flags=oracle.REF_FWD_FLAGS)
+    assert _relerr(got, ref) < 1e-5, (name, _relerr(got, ref))
+
+
+@pytest.mark.parametrize("gi", [0, 1, 2])
+def test_adjoint_exact_vs_fp64_transpose(oracle, gi):
+    name, vol, det = _geoms()[gi]
+    nh, ang, img, ov, od = _case(oracle, vol, det, 24, seed=gi + 3)
+    rng = np.random.default_rng(gi)
+    y = rng.uniform(0, 1, size=(24, det.nb, det.na)).astype(np.float32)
+    got = nh.backproject(torch.tensor(y, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
+    assert got.shape == ref.shape
+    assert _relerr(got, ref) < 1e-5, (name, _relerr(got, ref))
+
+
+@pytest.mark.parametrize("gi", [0, 2])
+def test_adjoint_reference_mode_vs_restatement(oracle, gi):
+    name, vol, det = _geoms()[gi]
+    nh, ang, img, ov, od = _case(oracle, vol, det, 24, seed=gi + 5)
+    rng = np.random.default_rng(gi + 10)
+    y = rng.uniform(0, 1, size=(24, det.nb, det.na)).astype(np.float32)
+    got = nh.backproject(torch.tensor(y, device=DEV), vol, det, torch.tensor(ang), flags=nh.REF_ADJOINT).cpu().numpy()
+    ref = oracle.ref_bp(ov, od, ang, y, flags=oracle.REF_BP_FLAGS)
+    assert _relerr(got, ref) < 1e-5, (name, _relerr(got, ref))
+
+
+@pytest.mark.parametrize("gi", [0, 1, 2])
+def test_dot_test_gpu(gi):
+    nh = _nh()
+    name, vol, det = _geoms()[gi]
+    g = torch.Generator(device="cpu").manual_seed(gi)
+    nv = 20
+    ang = torch.rand(nv, generator=g) * 2 * math.pi
+    x = torch.randn(vol.shape_zyx, generator=g).to(DEV)
+    y = torch.randn((nv, det.nb, det.na), generator=g).to(DEV)
+    Ax = nh.project(x, vol, det, ang)
+    ATy = nh.backproject(y, vol, det, ang)
+    lhs = torch.sum(Ax.double() * y.double()).item()
+    rhs = torch.sum(x.double() * ATy.double()).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs), (lhs, rhs)
+
+
+def test_count_vs_oracle(oracle):
+    name, vol, det = _geoms()[0]
+    nh, ang, img, ov, od = _case(oracle, vol, det, 12)
+    got = nh.count_intersections(vol, det, torch.tensor(ang)).cpu().numpy()
+    total, per = oracle.exact_count(ov, od, ang.astype(np.float64), per_ray=True)
+    assert abs(int(got.sum()) - total) <= 1e-5 * total
+    assert np.mean(got != per) < 1e-3
+
+
+def test_steep_rays_z_dominant(oracle):
+    """Detector taller than wide at short distance: many rays step in z."""
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    vol = VolumeGeometry.cube((24, 24, 24), 1.0)
+    det = ConeGeometry.flat(30.0, 45.0, 12, 4.0, 40, 6.0)
+    nh, ang, img, ov, od = _case(oracle, vol, det, 6, k=5)
+    got = nh.project(img.to(DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_fwd(ov, od, ang.astype(np.float64), img.double().numpy())
+    assert _relerr(got, ref) < 1e-5
+    y = np.random.default_rng(1).uniform(size=ref.shape).astype(np.float32)
+    gb = nh.backproject(torch.tensor(y, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    rb = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
+    assert _relerr(gb, rb) < 1e-5
+
+
+def test_edge_cases_missing_rays_and_ragged_detector(oracle):
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    vol = VolumeGeometry.cube((16, 16, 16), 1.0)
+    # detector much wider than the volume shadow: most rays miss; na, nb not tile multiples
+    det = ConeGeometry.flat(100.0, 150.0, 67, 3.0, 5, 3.0)
+    nh, ang, img, ov, od = _case(oracle, vol, det, 3, k=4)
+    got = nh.project(img.to(DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_fwd(ov, od, ang.astype(np.float64), img.double().numpy())
+    assert np.all(got[ref == 0] == 0)
+    assert _relerr(got, ref) < 1e-5
+    # single view, single row
+    det1 = ConeGeometry.flat(100.0, 150.0, 9, 1.0, 1, 1.0)
+    got1 = nh.project(img.to(DEV), vol, det1, torch.tensor([0.3])).cpu().numpy()
+    od1 = oracle.det_struct(0, 100.0, 150.0, 9, 1.0, 0.5, 1, 1.0, 0.5)
+    ref1 = oracle.exact_fwd(ov, od1, [np.float64(np.float32(0.3))], img.double().numpy())
+    assert _relerr(got1, ref1) < 1e-5
+
+
+def test_bad_arguments_raise():
+    nh = _nh()
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    vol = VolumeGeometry.cube((8, 8, 8), 1.0)
+    det = ConeGeometry.flat(100.0, 150.0, 4, 1.0, 4, 1.0)
+    x = torch.zeros(vol.shape_zyx, device=DEV)
+    with pytest.raises(nh.NefctError):
+        nh.project(x, vol, ConeGeometry.flat(100.0, 150.0, 4, float("nan"), 4, 1.0), [0.0])
+    with pytest.raises(nh.NefctError):
+        nh.project(x[:4], vol, det, [0.0])
+    with pytest.raises(nh.NefctError):
+        nh.project(x.cpu(), vol, det, [0.0])
+    aniso = VolumeGeometry((8, 8, 8), (0, 0, 0), (8.0, 16.0, 8.0))
+    with pytest.raises(nh.NefctError):
+        nh.project(x, aniso, det, [0.0], flags=nh.REF_FORWARD)
+
+
+def test_full_size_c2_subset_properties():
+    """At the C2 volume/detector size on 6 views: project(1) equals the ray's
+    chord with the box (computed in FP64 by the slab method), and the dot test."""
+    nh = _nh()
+    from nefct_hip.configs import c2
+    wl = c2()
+    vol, det = wl.vol, wl.det
+    ang = torch.tensor(wl.angles()[::120].copy())
+    ones = torch.ones(vol.shape_zyx, device=DEV)
+    p = nh.project(ones, vol, det, ang).double()
+    # chords in FP64
+    nv = ang.numel()
+    a = (det.ai + torch.arange(det.na, dtype=torch.float64, device=DEV)) * det.da - det.na * det.da / 2
+    b = (det.bi + torch.arange(det.nb, dtype=torch.float64, device=DEV)) * det.db - det.nb * det.db / 2
+    t = ang.double().to(DEV)
+    c, s = torch.cos(t)[:, None, None], torch.sin(t)[:, None, None]
+    B, A = torch.meshgrid(b, a, indexing="ij")
+    A, B = A[None], B[None]
+    S = torch.stack([-det.SID * c + 0 * A, -det.SID * s + 0 * A, 0 * A + 0 * c])
+    D = torch.stack([(det.SAD - det.SID) * c - A * s, (det.SAD - det.SID) * s + A * c, B + 0 * c])
+    d = D - S
+    lo = torch.tensor([-s_ / 2 for s_ in vol.size], dtype=torch.float64, device=DEV)
+    tin = torch.full_like(A + 0 * c, -1e300)
+    tout = torch.full_like(A + 0 * c, 1e300)
+    for i in range(3):
+        t0 = (lo[i] - S[i]) / d[i]
+        t1 = (-lo[i] - S[i]) / d[i]
+        tin = torch.maximum(tin, torch.minimum(t0, t1))
+        tout = torch.minimum(tout, torch.maximum(t0, t1))
+    chord = torch.clamp(tout - tin, min=0) * torch.sqrt((d * d).sum(0))
+    assert (torch.norm(p - chord) / torch.norm(chord)).item() < 1e-5
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x = torch.randn(vol.shape_zyx, generator=g).to(DEV)
+    y = torch.randn((nv, det.nb, det.na), generator=g).to(DEV)
+    lhs = torch.sum(nh.project(x, vol, det, ang).double() * y.double()).item()
+    rhs = torch.sum(x.double() * nh.backproject(y, vol, det, ang).double()).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+def test_autograd_matches_adjoint():
+    nh = _nh()
+    name, vol, det = _geoms()[0]
+    ang = torch.linspace(0, 3.0, 10)
+    x = torch.rand(vol.shape_zyx, device=DEV, requires_grad=True)
+    y = torch.rand((10, det.nb, det.na), device=DEV)
+    loss = (nh.forward_project(x, vol, det, ang) * y).sum()
+    loss.backward()
+    ref = nh.backproject(y, vol, det, ang)
+    assert torch.allclose(x.grad, ref, rtol=1e-5, atol=1e-6 * ref.abs().max().item())

@@ -1,0 +1,283 @@
+"""Pins the CPU oracle's Siddon functions against mathematics that fixes them.
+
+The reference ships no tests or fixtures for the projector (its tests cover the
+object model only), and its CUDA/TensorFlow op cannot be built here, so each
+oracle function is pinned by an independent check:
+
+* exact_fwd: closed-form chord of a ray with a box (slab method, computed here
+  in a few lines of numpy), brute-force supersampled line integrals on tiny
+  volumes, axis-aligned rays through known voxels, rotation symmetry.
+* exact_bp: dot test against exact_fwd (it is the transpose).
+* exact_count: hand-countable rays.
+* ref_fwd / ref_bp (FP32 restatement of nefct/cc/siddon_3d/siddon.cu.cc): with
+  the quirks cleared it must equal the FP64 oracle to FP32 precision; the Q1
+  factor must be exactly 1/L^2; the Q2 attribution must differ from exact only
+  on rays that hold a slab with both a y and a z crossing of opposite slope
+  signs, and equal it elsewhere (this is the verification that Q2 is real).
+"""
+import numpy as np
+import pytest
+
+
+def slab_chord(S, D, lo, hi):
+    """Length (mm) of the line through S, D inside the box [lo, hi]."""
+    d = D - S
+    t0, t1 = -np.inf, np.inf
+    for i in range(3):
+        if d[i] == 0:
+            if not (lo[i] < S[i] < hi[i]):
+                return 0.0
+            continue
+        a, b = (lo[i] - S[i]) / d[i], (hi[i] - S[i]) / d[i]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    return max(0.0, t1 - t0) * np.linalg.norm(d)
+
+
+def brute_line_integral(img, lo, vs, S, D, nsamp=400000):
+    """Midpoint Riemann sum of the piecewise-constant image along the line."""
+    d = D - S
+    n = np.array(img.shape[::-1])
+    hi = lo + n * vs
+    t0, t1 = -np.inf, np.inf
+    for i in range(3):
+        a, b = (lo[i] - S[i]) / d[i], (hi[i] - S[i]) / d[i]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    if t1 <= t0:
+        return 0.0
+    t = t0 + (np.arange(nsamp) + 0.5) * (t1 - t0) / nsamp
+    P = S[None, :] + t[:, None] * d[None, :]
+    idx = np.floor((P - lo) / vs).astype(int)
+    idx = np.clip(idx, 0, n - 1)
+    vals = img[idx[:, 2], idx[:, 1], idx[:, 0]]
+    return vals.sum() * (t1 - t0) / nsamp * np.linalg.norm(d)
+
+
+def test_single_voxel_chord(oracle):
+    rng = np.random.default_rng(1)
+    V = oracle.vol_struct((1, 1, 1), (0.3, -0.2, 0.1), (2.0, 3.0, 1.5))
+    lo = np.array([0.3 - 1.0, -0.2 - 1.5, 0.1 - 0.75])
+    hi = lo + np.array([2.0, 3.0, 1.5])
+    for _ in range(300):
+        S = rng.normal(size=3) * 5
+        D = rng.normal(size=3) * 0.5
+        got = oracle.exact_ray(V, np.ones((1, 1, 1)), S, D)
+        assert abs(got - slab_chord(S, D, lo, hi)) < 1e-12 * max(1.0, abs(got))
+
+
+def test_uniform_volume_equals_box_chord(oracle):
+    rng = np.random.default_rng(2)
+    shape = (7, 5, 9)
+    size = np.array([7 * 1.3, 5 * 0.9, 9 * 0.7])
+    c = np.array([0.5, -0.3, 0.2])
+    V = oracle.vol_struct(shape, c, size)
+    img = np.full(shape[::-1], 2.5)
+    for _ in range(300):
+        S = rng.normal(size=3) * 20
+        D = rng.normal(size=3) * 2
+        got = oracle.exact_ray(V, img, S, D)
+        n, tot = oracle.exact_ray_count(V, S, D)
+        chord = slab_chord(S, D, c - size / 2, c + size / 2)
+        assert abs(got - 2.5 * chord) < 1e-10
+        assert abs(tot - chord) < 1e-10
+
+
+def test_brute_force_random_image(oracle):
+    rng = np.random.default_rng(3)
+    shape = (6, 5, 4)
+    vs = np.array([1.0, 1.2, 0.8])
+    size = vs * np.array(shape)
+    c = np.array([0.2, 0.1, -0.3])
+    V = oracle.vol_struct(shape, c, size)
+    img = rng.uniform(0, 1, size=shape[::-1])
+    for _ in range(12):
+        S = rng.normal(size=3) * 15
+        D = -S * 0.6 + rng.normal(size=3)
+        got = oracle.exact_ray(V, img, S, D)
+        bf = brute_line_integral(img, c - size / 2, vs, S, D)
+        assert abs(got - bf) < 2e-4 * max(1.0, abs(bf))
+
+
+def test_axis_aligned_ray_hits_known_voxels(oracle):
+    # a ray along +x at the centre of voxel row (iy=2, iz=1) integrates that row
+    shape = (8, 4, 3)
+    V = oracle.vol_struct(shape, (0, 0, 0), shape)
+    img = np.zeros(shape[::-1])
+    img[1, 2, :] = np.arange(8) + 1.0
+    S = np.array([-50.0, -2 + 2.5, -1.5 + 1.5])
+    D = np.array([50.0, -2 + 2.5, -1.5 + 1.5])
+    assert oracle.exact_ray(V, img, S, D) == pytest.approx(36.0, abs=1e-12)
+    n, tot = oracle.exact_ray_count(V, S, D)
+    assert n == 8 and tot == pytest.approx(8.0)
+    # a 45-degree in-plane diagonal through voxel corners: 4 voxels, chord 4*sqrt(2)
+    V2 = oracle.vol_struct((4, 4, 1), (0, 0, 0), (4, 4, 1))
+    n, tot = oracle.exact_ray_count(V2, np.array([-10.0, -10.0, 0.0]), np.array([10.0, 10.0, 0.0]))
+    assert n == 4 and tot == pytest.approx(4 * np.sqrt(2))
+
+
+def test_geometry_central_ray_and_rotation(oracle):
+    det = oracle.det_struct(0, 1000.0, 1500.0, 5, 1.5, 0.5, 5, 1.5, 0.5)
+    S, D = oracle.ray_points(det, 0.0, 2, 2)
+    assert np.allclose(S, [-1000, 0, 0]) and np.allclose(D, [500, 0, 0])
+    S, D = oracle.ray_points(det, np.pi / 2, 2, 2)
+    assert np.allclose(S, [0, -1000, 0], atol=1e-9) and np.allclose(D, [0, 500, 0], atol=1e-9)
+    # centred sphere: sinogram identical across views and symmetric in a, b
+    n = 16
+    V = oracle.vol_struct((n, n, n), (0, 0, 0), (n, n, n))
+    g = np.arange(n) - n / 2 + 0.5
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    img = ((X ** 2 + Y ** 2 + Z ** 2) < 36).astype(float)
+    det = oracle.det_struct(0, 60.0, 90.0, 12, 2.0, 0.5, 12, 2.0, 0.5)
+    p = oracle.exact_fwd(V, det, [0.0, np.pi / 2, np.pi], img)
+    assert np.allclose(p[0], p[1], atol=1e-9) and np.allclose(p[0], p[2], atol=1e-9)
+    assert np.allclose(p[0], p[0][::-1, :], atol=1e-9) and np.allclose(p[0], p[0][:, ::-1], atol=1e-9)
+
+
+def test_exact_bp_is_transpose(oracle):
+    rng = np.random.default_rng(4)
+    shape = (10, 9, 7)
+    V = oracle.vol_struct(shape, (0.4, -0.3, 0.2), (10.0, 9.0, 8.4))
+    det = oracle.det_struct(0, 40.0, 70.0, 11, 1.7, 0.5, 9, 1.9, 0.3)
+    ang = rng.uniform(0, 2 * np.pi, 5)
+    x = rng.normal(size=shape[::-1])
+    y = rng.normal(size=(5, 9, 11))
+    lhs = np.vdot(oracle.exact_fwd(V, det, ang, x), y)
+    rhs = np.vdot(x, oracle.exact_bp(V, det, ang, y))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def _phantom(shape, rng, k=6):
+    nz, ny, nx = shape
+    Z, Y, X = np.meshgrid(np.linspace(-1, 1, nz), np.linspace(-1, 1, ny), np.linspace(-1, 1, nx), indexing="ij")
+    img = np.zeros(shape)
+    for _ in range(k):
+        c = rng.uniform(-0.4, 0.4, 3)
+        ax = rng.uniform(0.15, 0.6, 3)
+        img += rng.uniform(0.5, 1.5) * ((((X - c[0]) / ax[0]) ** 2 + ((Y - c[1]) / ax[1]) ** 2
+                                        + ((Z - c[2]) / ax[2]) ** 2) <= 1)
+    return img
+
+
+@pytest.mark.parametrize("cyl", [0, 1])
+def test_ref_restatement_corrected_equals_exact(oracle, cyl):
+    rng = np.random.default_rng(5)
+    shape = (24, 24, 20)
+    V = oracle.vol_struct(shape, (0.0, 0.0, 0.0), (24.0, 24.0, 20.0))
+    if cyl:
+        det = oracle.det_struct(1, 60.0, 100.0, 30, 0.012, 0.5, 24, 1.4, 0.5)
+    else:
+        det = oracle.det_struct(0, 60.0, 100.0, 30, 1.5, 0.5, 24, 1.4, 0.5)
+    ang = np.linspace(0, 2 * np.pi, 13, endpoint=False) + 0.07
+    img = _phantom(shape[::-1], rng)
+    ex = oracle.exact_fwd(V, det, ang, img)
+    rf = oracle.ref_fwd(V, det, ang.astype(np.float32), img.astype(np.float32), flags=0)
+    rel = np.linalg.norm(rf - ex) / np.linalg.norm(ex)
+    assert rel < 1e-5, rel
+
+
+def test_q1_inv_l2_factor(oracle):
+    rng = np.random.default_rng(6)
+    shape = (16, 16, 12)
+    V = oracle.vol_struct(shape, (0, 0, 0), (16 * 0.8, 16 * 0.8, 12 * 0.8))
+    det = oracle.det_struct(0, 50.0, 80.0, 10, 1.2, 0.5, 8, 1.2, 0.5)
+    ang = np.array([0.1, 1.0, 2.0], np.float32)
+    img = _phantom(shape[::-1], rng).astype(np.float32)
+    a = oracle.ref_fwd(V, det, ang, img, flags=oracle.REF_SLAB)
+    b = oracle.ref_fwd(V, det, ang, img, flags=oracle.REF_SLAB | oracle.INV_L2)
+    for iv, t in enumerate(ang):
+        for ib in range(8):
+            for ia in range(10):
+                S, D = oracle.ray_points(det, float(t), ia, ib)
+                L = np.linalg.norm(D - S) / 0.8
+                assert b[iv, ib, ia] == pytest.approx(a[iv, ib, ia] / L ** 2, rel=2e-6, abs=1e-12)
+
+
+def _has_mixed_double_crossing(S, D, shape, size):
+    """True if the x- (or y-) slab walk of the ray has a slab with both a
+    transverse crossing and a z crossing whose slopes have opposite signs."""
+    nx, ny, nz = shape
+    vs = np.array(size) / np.array(shape)
+    lo = -np.array(size) / 2
+    d = D - S
+    dom = 0 if abs(d[0]) >= abs(d[1]) else 1
+    tr = 1 - dom
+    for k in range(shape[dom]):
+        f = []
+        for face in (k, k + 1):
+            t = (lo[dom] + face * vs[dom] - S[dom]) / d[dom]
+            P = S + t * d
+            f.append(((P[tr] - lo[tr]) / vs[tr], (P[2] - lo[2]) / vs[2]))
+        cy = np.floor(f[0][0]) != np.floor(f[1][0])
+        cz = np.floor(f[0][1]) != np.floor(f[1][1])
+        inside = -1 <= min(f[0][0], f[1][0]) < shape[tr] and -1 <= min(f[0][1], f[1][1]) < nz
+        if cy and cz and inside and (d[tr] * d[2] < 0):
+            return True
+    return False
+
+
+def test_q2_reference_attribution_verified(oracle):
+    """Q2 is real: the reference attribution differs from exact exactly on rays
+    holding a mixed-sign double-crossing slab, and nowhere else."""
+    rng = np.random.default_rng(7)
+    shape = (20, 20, 16)
+    size = (20.0, 20.0, 16.0)
+    V = oracle.vol_struct(shape, (0, 0, 0), size)
+    det = oracle.det_struct(0, 40.0, 70.0, 16, 2.1, 0.5, 16, 2.3, 0.5)
+    ang = np.array([0.3, 1.2, 2.6, 4.0], np.float32)
+    img = rng.uniform(0, 1, size=shape[::-1]).astype(np.float32)
+    q2 = oracle.ref_fwd(V, det, ang, img, flags=oracle.REF_SLAB)
+    ok = oracle.ref_fwd(V, det, ang, img, flags=0)
+    n_mixed = 0
+    for iv, t in enumerate(ang):
+        for ib in range(16):
+            for ia in range(16):
+                S, D = oracle.ray_points(det, float(t), ia, ib)
+                mixed = _has_mixed_double_crossing(S, D, shape, size)
+                diff = abs(q2[iv, ib, ia] - ok[iv, ib, ia])
+                scale = max(abs(ok[iv, ib, ia]), 1e-3)
+                if mixed:
+                    n_mixed += 1
+                else:
+                    assert diff <= 1e-6 * scale
+    assert n_mixed > 50
+    # the differing rays carry a clearly visible error on a random image
+    assert np.linalg.norm(q2 - ok) / np.linalg.norm(ok) > 1e-3
+
+
+def test_ref_bp_is_transpose_of_ref_fwd(oracle):
+    rng = np.random.default_rng(8)
+    shape = (14, 14, 10)
+    V = oracle.vol_struct(shape, (0, 0, 0), (14.0, 14.0, 10.0))
+    det = oracle.det_struct(0, 45.0, 75.0, 12, 1.9, 0.5, 10, 1.9, 0.5)
+    ang = rng.uniform(0, 2 * np.pi, 6).astype(np.float32)
+    x = rng.normal(size=shape[::-1]).astype(np.float32)
+    y = rng.normal(size=(6, 10, 12)).astype(np.float32)
+    for flags in (0, oracle.REF_SLAB):
+        Ax = oracle.ref_fwd(V, det, ang, x, flags=flags).astype(np.float64)
+        ATy = oracle.ref_bp(V, det, ang, y, flags=flags).astype(np.float64)
+        lhs, rhs = np.vdot(Ax, y), np.vdot(x.astype(np.float64), ATy)
+        assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_ref_pair_is_not_adjoint_q1(oracle):
+    """The reference pair (forward with 1/L^2, adjoint without) fails the dot test."""
+    rng = np.random.default_rng(9)
+    shape = (12, 12, 8)
+    V = oracle.vol_struct(shape, (0, 0, 0), (12.0, 12.0, 8.0))
+    det = oracle.det_struct(0, 45.0, 75.0, 10, 1.9, 0.5, 8, 1.9, 0.5)
+    ang = rng.uniform(0, 2 * np.pi, 4).astype(np.float32)
+    x = rng.uniform(size=shape[::-1]).astype(np.float32)
+    y = rng.uniform(size=(4, 8, 10)).astype(np.float32)
+    lhs = np.vdot(oracle.ref_fwd(V, det, ang, x).astype(np.float64), y)
+    rhs = np.vdot(x.astype(np.float64), oracle.ref_bp(V, det, ang, y).astype(np.float64))
+    assert abs(lhs - rhs) > 0.5 * abs(rhs)
+
+
+def test_exact_count_matches_per_ray(oracle):
+    shape = (9, 8, 7)
+    V = oracle.vol_struct(shape, (0, 0, 0), (9.0, 8.0, 7.0))
+    det = oracle.det_struct(0, 30.0, 50.0, 6, 2.0, 0.5, 5, 2.0, 0.5)
+    ang = np.array([0.2, 0.9])
+    total, per = oracle.exact_count(V, det, ang, per_ray=True)
+    assert total == int(per.sum()) and total > 0
+    S, D = oracle.ray_points(det, 0.9, 3, 2)
+    assert per[1, 2, 3] == oracle.exact_ray_count(V, S, D)[0]
