@@ -583,6 +583,163 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint owned by volume tiles (no global atomics).
+//
+// A workgroup owns a TX x TY x TZ block of voxels held in LDS.  For every view
+// it enumerates the detector cells whose rays can reach the block (the
+// block's shadow, from its 8 corners), sets each ray up exactly as the forward
+// kernel does, walks only the slabs where the ray is within one cell of the
+// block, and adds each segment that lands inside the block to LDS.  After
+// the last view the block is written once.  Every (ray, slab, cell) weight is
+// the forward kernel's own arithmetic, and every voxel belongs to one block,
+// so the result is the transpose of the forward up to summation order.
+// With one wavefront per block (NW = 1) the LDS summation order is fixed, so
+// the result is bitwise reproducible.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int TZ>
+struct TileAcc {
+    static constexpr int SX = TX + 1;  // odd row stride: lanes on consecutive y rows hit distinct banks
+    float v[TZ][TY][SX];
+};
+
+__device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
+                                              const int hi[3], int& ia0, int& ia1, int& ib0, int& ib1)
+{
+    // fractional detector indices of the 8 corners (flat: a = t SDD / U, b = z SDD / U;
+    // cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)); margin of 1 cell.
+    float fa_mn = 1e30f, fa_mx = -1e30f, fb_mn = 1e30f, fb_mx = -1e30f;
+    bool behind = false;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double x = V.lo[0] + V.vs[0] * ((q & 1) ? hi[0] : lo[0]);
+        const double y = V.lo[1] + V.vs[1] * ((q & 2) ? hi[1] : lo[1]);
+        const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]);
+        const double U = G.sid + x * c + y * s;
+        const double t = -x * s + y * c;
+        if (U <= 1e-6 * G.sid) behind = true;
+        double a, b;
+        if (G.cyl) {
+            a = atan2(t, U);
+            b = z * G.sad / sqrt(U * U + t * t);
+        } else {
+            a = t * G.sad / U;
+            b = z * G.sad / U;
+        }
+        const float fa = (float)((a + G.na * G.da * 0.5) / G.da - G.ai);
+        const float fb = (float)((b + G.nb * G.db * 0.5) / G.db - G.bi);
+        fa_mn = fminf(fa_mn, fa);
+        fa_mx = fmaxf(fa_mx, fa);
+        fb_mn = fminf(fb_mn, fb);
+        fb_mx = fmaxf(fb_mx, fb);
+    }
+    if (behind) {  // block reaches behind the source: take the whole detector
+        ia0 = 0; ia1 = G.na - 1; ib0 = 0; ib1 = G.nb - 1;
+        return;
+    }
+    ia0 = max(0, (int)floorf(fa_mn) - 1);
+    ia1 = min(G.na - 1, (int)ceilf(fa_mx) + 1);
+    ib0 = max(0, (int)floorf(fb_mn) - 1);
+    ib1 = min(G.nb - 1, (int)ceilf(fb_mx) + 1);
+}
+
+template <bool REF, int TX, int TY, int TZ, int NW>
+__global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
+                                                             float* __restrict__ image)
+{
+    __shared__ TileAcc<TX, TY, TZ> acc;
+    constexpr int SX = TileAcc<TX, TY, TZ>::SX;
+    const int tid = threadIdx.x;
+    float* af = &acc.v[0][0][0];
+    for (int i = tid; i < TZ * TY * SX; i += 64 * NW) af[i] = 0.0f;
+    const int nx = P.V.n[0], ny = P.V.n[1], nz = P.V.n[2];
+    const int b0[3] = {(int)blockIdx.x * TX, (int)blockIdx.y * TY, (int)blockIdx.z * TZ};
+    const int ext[3] = {min(TX, nx - b0[0]), min(TY, ny - b0[1]), min(TZ, nz - b0[2])};
+    const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
+    __syncthreads();
+    for (int iv = 0; iv < P.nv; ++iv) {
+        const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
+        int ia0, ia1, ib0, ib1;
+        shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
+        if (ia1 < ia0 || ib1 < ib0) continue;
+        const int wa = ia1 - ia0 + 1;
+        const int ncand = wa * (ib1 - ib0 + 1);
+        for (int k = tid; k < ncand; k += 64 * NW) {
+            const int ib = ib0 + k / wa, ia = ia0 + k % wa;
+            Ray r;
+            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
+            if (r.dom < 0) continue;
+            const int m = r.dom, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
+            // slabs of the block on the walk axis, within the ray's own range
+            int t0 = max(r.s0, b0[m]), t1 = min(r.s1, b1[m]);
+            if (t0 >= t1) continue;
+            // FP32 coordinates of the ray at the block's slab faces decide the
+            // transverse overlap (one cell of margin), as the forward would see them
+            const float fa0 = fmaf(r.ka, (float)(t0 - r.tc), r.Ac), fa1 = fmaf(r.ka, (float)(t1 - r.tc), r.Ac);
+            const float fb0 = fmaf(r.kb, (float)(t0 - r.tc), r.Bc), fb1 = fmaf(r.kb, (float)(t1 - r.tc), r.Bc);
+            const int ha = P.V.n[a] >> 1, hb = P.V.n[b] >> 1;
+            const float alo = (float)(b0[a] - ha - 1), ahi = (float)(b1[a] - ha + 1);
+            const float blo = (float)(b0[b] - hb - 1), bhi = (float)(b1[b] - hb + 1);
+            if (fmaxf(fa0, fa1) < alo || fminf(fa0, fa1) > ahi || fmaxf(fb0, fb1) < blo || fminf(fb0, fb1) > bhi)
+                continue;
+            // tighten [t0, t1) to the slabs where the ray is within the margin box
+            if (r.ka != 0.0f) {
+                float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
+                if (ta > tb) { const float q = ta; ta = tb; tb = q; }
+                t0 = max(t0, (int)floorf(ta) - 1);
+                t1 = min(t1, (int)ceilf(tb) + 1);
+            }
+            if (r.kb != 0.0f) {
+                float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
+                if (ta > tb) { const float q = ta; ta = tb; tb = q; }
+                t0 = max(t0, (int)floorf(ta) - 1);
+                t1 = min(t1, (int)ceilf(tb) + 1);
+            }
+            if (t0 >= t1) continue;
+            const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
+            const float sv = pv[id] * r.scale;
+            const Walk w = walk_of(m, P.V, P.Ly);
+            float tf = (float)(t0 - r.tc);
+            float a0 = fmaf(r.ka, tf, r.Ac), bb0 = fmaf(r.kb, tf, r.Bc);
+            float fla0 = floorf(a0), flb0 = floorf(bb0);
+            for (int t = t0; t < t1; ++t) {
+                tf += 1.0f;
+                const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
+                const float fla1 = floorf(a1), flb1 = floorf(bb1);
+                Seg sg;
+                slab_segments<REF>(a0, a1, fla0, fla1, bb0, bb1, flb0, flb1, r, w, sg);
+                if (sg.ok) {
+                    const int ca[3] = {sg.a1, sg.a2, sg.a3};
+                    const int cb[3] = {sg.b1, sg.b2, sg.b3};
+                    const float wk[3] = {sg.w1, sg.w2, sg.w3};
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        if (!(wk[q] > 0.0f)) continue;
+                        int l[3];
+                        l[m] = t - b0[m];
+                        l[a] = ca[q] + ha - b0[a];
+                        l[b] = cb[q] + hb - b0[b];
+                        if ((unsigned)l[0] < (unsigned)ext[0] && (unsigned)l[1] < (unsigned)ext[1] &&
+                            (unsigned)l[2] < (unsigned)ext[2])
+                            atomicAdd(&acc.v[l[2]][l[1]][l[0]], wk[q] * sv);
+                    }
+                }
+                a0 = a1;
+                bb0 = bb1;
+                fla0 = fla1;
+                flb0 = flb1;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < TX * TY * TZ; i += 64 * NW) {
+        const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
+        if (lx < ext[0] && ly < ext[1] && lz < ext[2])
+            image[(size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz))] =
+                acc.v[lz][ly][lx];
+    }
+}
+
 __global__ void k_view_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -711,8 +868,10 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
         return NEFCT_ERR_GEOMETRY;
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
     if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
-    if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT))
+    if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT | NEFCT_F_DETERMINISTIC |
+                  NEFCT_F_ADJ_WINDOW))
         return NEFCT_ERR_UNSUPPORTED;
+    if ((flags & NEFCT_F_DETERMINISTIC) && (flags & NEFCT_F_ADJ_WINDOW)) return NEFCT_ERR_UNSUPPORTED;
     if (flags & NEFCT_F_REF_SLAB) {
         // the reference y-walk treats y slabs as x-voxel thick (siddon.cu.cc:188-201)
         const double vx = (double)size[0] / grid[0], vy = (double)size[1] / grid[1];
@@ -812,16 +971,34 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
     char* ws = (char*)workspace;
     Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
     launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
-    launch_zero((float*)(ws + p.xf_off), p.xf_elems, s);
-    launch_zero((float*)(ws + p.yf_off), p.yf_elems, s);
-    const dim3 g((na + BX - 1) / BX, (nb + BY * AR - 1) / (BY * AR), nv), b(BX, BY);
-    if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_siddon_adj<true>), g, b, 0, s, L, pv);
-    else
-        hipLaunchKernelGGL((k_siddon_adj<false>), g, b, 0, s, L, pv);
     const int nx = grid[0], ny = grid[1], nz = grid[2];
-    hipLaunchKernelGGL(k_combine, dim3((nx + 63) / 64, (ny + 63) / 64, nz), dim3(256), 0, s,
-                       (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP, image);
+    const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
+    if (flags & NEFCT_F_ADJ_WINDOW) {
+        // ray tiles with LDS slab windows flushed by global atomics
+        launch_zero((float*)(ws + p.xf_off), p.xf_elems, s);
+        launch_zero((float*)(ws + p.yf_off), p.yf_elems, s);
+        const dim3 g((na + BX - 1) / BX, (nb + BY * AR - 1) / (BY * AR), nv), b(BX, BY);
+        if (ref)
+            hipLaunchKernelGGL((k_siddon_adj<true>), g, b, 0, s, L, pv);
+        else
+            hipLaunchKernelGGL((k_siddon_adj<false>), g, b, 0, s, L, pv);
+        hipLaunchKernelGGL(k_combine, dim3((nx + 63) / 64, (ny + 63) / 64, nz), dim3(256), 0, s,
+                           (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP,
+                           image);
+    } else if (flags & NEFCT_F_DETERMINISTIC) {
+        // one wavefront per 16x16x8 block: fixed LDS summation order
+        const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + 7) / 8);
+        if (ref)
+            hipLaunchKernelGGL((k_siddon_adj_tile<true, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image);
+        else
+            hipLaunchKernelGGL((k_siddon_adj_tile<false, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image);
+    } else {
+        const dim3 g((nx + 31) / 32, (ny + 31) / 32, (nz + 7) / 8);
+        if (ref)
+            hipLaunchKernelGGL((k_siddon_adj_tile<true, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image);
+        else
+            hipLaunchKernelGGL((k_siddon_adj_tile<false, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image);
+    }
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }

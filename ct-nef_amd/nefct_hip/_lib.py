@@ -17,6 +17,8 @@ F_INV_L2 = 1
 F_REF_SLAB = 2
 F_SHORT_SKIP = 4
 F_TIE_GT = 8
+F_DETERMINISTIC = 16
+F_ADJ_WINDOW = 32
 REF_FORWARD = F_INV_L2 | F_REF_SLAB
 REF_ADJOINT = F_REF_SLAB | F_SHORT_SKIP | F_TIE_GT
 

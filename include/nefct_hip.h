@@ -51,6 +51,12 @@ typedef enum {
 #define NEFCT_F_REF_SLAB 2u   /* Q2: reference slab walk and attribution (siddon.cu.cc:41-176) */
 #define NEFCT_F_SHORT_SKIP 4u /* Q3: adjoint skips rays with in-plane length < 10 voxels (:472) */
 #define NEFCT_F_TIE_GT 8u     /* Q6: adjoint x-walk only if |xd| > |yd| (:480) */
+/* Adjoint algorithm selection (ignored by the forward):
+ *   default                 volume blocks own their voxels in LDS (no global atomics)
+ *   NEFCT_F_DETERMINISTIC   one wavefront per block: bitwise reproducible result
+ *   NEFCT_F_ADJ_WINDOW      ray tiles with LDS slab windows flushed by global atomics */
+#define NEFCT_F_DETERMINISTIC 16u
+#define NEFCT_F_ADJ_WINDOW 32u
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
 
