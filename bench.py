@@ -66,6 +66,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-stride", type=int, default=45)
+    ap.add_argument("--views", type=int, default=0,
+                    help="profiling only: project the first N views instead of all 720 (not the headline number)")
+    ap.add_argument("--adj-flags", type=int, default=0,
+                    help="profiling only: adjoint algorithm flags (16 deterministic, 32 slab windows)")
+    ap.add_argument("--only", choices=["pair", "fwd", "adj"], default="pair",
+                    help="profiling only: run one direction")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -84,7 +90,9 @@ def main():
     log(f"[bench] workload {wl.name}: building phantom")
     img = ellipsoid_phantom(vol, wl.seed, wl.ellipsoids, device=dev)
     ang = torch.tensor(wl.angles(), device=dev)
-    rays = wl.rays
+    if args.views:
+        ang = ang[: args.views].contiguous()
+    rays = ang.numel() * det.na * det.nb
 
     log("[bench] counting intersections")
     counts = nh.count_intersections(vol, det, ang, device=dev)
@@ -98,11 +106,13 @@ def main():
         e1 = torch.cuda.Event(enable_timing=True)
         e2 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        sino = nh.project(img, vol, det, ang)
+        sino = nh.project(img, vol, det, ang) if args.only != "adj" else sino_fixed
         e1.record()
-        back = nh.backproject(sino, vol, det, ang)
+        back = nh.backproject(sino, vol, det, ang, flags=args.adj_flags) if args.only != "fwd" else None
         e2.record()
         return e0, e1, e2, sino, back
+
+    sino_fixed = nh.project(img, vol, det, ang) if args.only == "adj" else None
 
     for w in range(args.warmup):
         step()
@@ -119,8 +129,8 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     total_s = t1 - t0
-    t_fwd = sum(e0.elapsed_time(e1) for e0, e1, _ in evs) / 1e3 / args.steps
-    t_adj = sum(e1.elapsed_time(e2) for _, e1, e2 in evs) / 1e3 / args.steps
+    t_fwd = max(sum(e0.elapsed_time(e1) for e0, e1, _ in evs) / 1e3 / args.steps, 1e-9)
+    t_adj = max(sum(e1.elapsed_time(e2) for _, e1, e2 in evs) / 1e3 / args.steps, 1e-9)
     log(f"[bench] timed {args.steps} steps: {total_s:.3f} s (fwd {t_fwd * 1e3:.1f} ms, adj {t_adj * 1e3:.1f} ms)")
 
     value = rays * args.steps / total_s / 1e9
@@ -151,7 +161,7 @@ def main():
         "config": {
             "workload": wl.name,
             "volume": list(vol.grid), "voxel_mm": vol.voxel[0],
-            "detector": [det.na, det.nb], "pitch_mm": det.da, "views": wl.nv,
+            "detector": [det.na, det.nb], "pitch_mm": det.da, "views": int(ang.numel()),
             "sod_mm": det.SID, "sdd_mm": det.SAD, "rays": rays, "sum_n_int": n_int,
             "parallelism": "single GPU",
         },

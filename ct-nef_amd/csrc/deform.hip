@@ -1,0 +1,213 @@
+// Trilinear volume warp by a dense displacement field, its transpose and its
+// gradient with respect to the displacement, for gfx950.
+//
+// Replaces cpp/deform/deform.cu.cc:13-46 (DeformKernel) and :112-126 (host
+// launcher) behind the Deform op (cpp/deform/deform.cc:12-22, :59-91), with the
+// zero-filled output of nefct/cc/deform/deform.cc:94.  The reference registers
+// no gradient; the transpose (scatter) and the displacement gradient here make
+// the warp differentiable.
+//
+//   out[i] = sum over the 8 corners c of img[c] * w_c(p_i),  p_i = i + m(i)
+//
+// m is in voxel units.  Boundary rule 0 (reference): the voxel is written only
+// when every base index floor(p) lies in [1, n-2] (deform.cu.cc:31-33), else 0.
+// Rule 1 (zero padding): every corner inside the volume contributes.
+// One lane per output voxel, x fastest: the three displacement reads and the
+// write are coalesced, the 8 gathers hit L2 (neighbouring lanes share corners).
+#include "common.h"
+
+namespace {
+
+struct Corners {
+    int x0, y0, z0;
+    float fx, fy, fz;   // weights of the upper corners
+    bool ok;            // reference rule: all base indices in [1, n-2]
+};
+
+__device__ __forceinline__ Corners corners_of(int ix, int iy, int iz, float mx, float my, float mz, int nx, int ny,
+                                              int nz)
+{
+    Corners c;
+    const float px = (float)ix + mx, py = (float)iy + my, pz = (float)iz + mz;
+    const float flx = floorf(px), fly = floorf(py), flz = floorf(pz);
+    c.x0 = (int)flx;
+    c.y0 = (int)fly;
+    c.z0 = (int)flz;
+    c.fx = px - flx;
+    c.fy = py - fly;
+    c.fz = pz - flz;
+    c.ok = c.x0 > 0 && c.x0 + 1 < nx && c.y0 > 0 && c.y0 + 1 < ny && c.z0 > 0 && c.z0 + 1 < nz;
+    return c;
+}
+
+__device__ __forceinline__ bool inside(int x, int y, int z, int nx, int ny, int nz)
+{
+    return (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz;
+}
+
+// Finite-precision guard: a non-finite displacement makes every corner invalid.
+__device__ __forceinline__ bool finite3(float a, float b, float c)
+{
+    return fabsf(a) < 1e30f && fabsf(b) < 1e30f && fabsf(c) < 1e30f;
+}
+
+__global__ void k_deform(const float* __restrict__ img, const float* __restrict__ mx, const float* __restrict__ my,
+                         const float* __restrict__ mz, int nx, int ny, int nz, int boundary, float* __restrict__ out)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t id = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz);
+    const float dx = mx[id], dy = my[id], dz = mz[id];
+    float v = 0.0f;
+    if (finite3(dx, dy, dz)) {
+        const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
+        if (boundary != 0 || c.ok) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int x = c.x0 + (q & 1), y = c.y0 + ((q >> 1) & 1), z = c.z0 + (q >> 2);
+                const float w = ((q & 1) ? c.fx : 1.0f - c.fx) * (((q >> 1) & 1) ? c.fy : 1.0f - c.fy) *
+                                ((q >> 2) ? c.fz : 1.0f - c.fz);
+                if (inside(x, y, z, nx, ny, nz))
+                    v = fmaf(img[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)], w, v);
+            }
+        }
+    }
+    out[id] = v;
+}
+
+// grad_img[c] += g[i] * w_c(p_i)   (transpose of k_deform in the image)
+__global__ void k_deform_adjoint(const float* __restrict__ g, const float* __restrict__ mx,
+                                 const float* __restrict__ my, const float* __restrict__ mz, int nx, int ny, int nz,
+                                 int boundary, float* __restrict__ gimg, uint32_t gbytes)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t id = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz);
+    const float gv = g[id];
+    const float dx = mx[id], dy = my[id], dz = mz[id];
+    if (gv == 0.0f || !finite3(dx, dy, dz)) return;
+    const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
+    if (boundary == 0 && !c.ok) return;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(gimg, gbytes);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int x = c.x0 + (q & 1), y = c.y0 + ((q >> 1) & 1), z = c.z0 + (q >> 2);
+        const float w = ((q & 1) ? c.fx : 1.0f - c.fx) * (((q >> 1) & 1) ? c.fy : 1.0f - c.fy) *
+                        ((q >> 2) ? c.fz : 1.0f - c.fz);
+        if (inside(x, y, z, nx, ny, nz) && w != 0.0f)
+            buf_atomic_add(rs, (int)(((size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)) * 4), gv * w);
+    }
+}
+
+// d<g, warp(img, m)>/dm at each voxel (the derivative of the trilinear
+// interpolant on the cell that contains p; zero where the reference rule
+// writes 0, since the output is constant there).
+__global__ void k_deform_dvf_grad(const float* __restrict__ img, const float* __restrict__ g,
+                                  const float* __restrict__ mx, const float* __restrict__ my,
+                                  const float* __restrict__ mz, int nx, int ny, int nz, int boundary,
+                                  float* __restrict__ gx, float* __restrict__ gy, float* __restrict__ gz)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= nx) return;
+    const size_t id = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz);
+    const float gv = g[id];
+    const float dx = mx[id], dy = my[id], dz = mz[id];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (gv != 0.0f && finite3(dx, dy, dz)) {
+        const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
+        if (boundary != 0 || c.ok) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int x = c.x0 + (q & 1), y = c.y0 + ((q >> 1) & 1), z = c.z0 + (q >> 2);
+                if (!inside(x, y, z, nx, ny, nz)) continue;
+                const float v = img[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+                const float wx = (q & 1) ? c.fx : 1.0f - c.fx, sx = (q & 1) ? 1.0f : -1.0f;
+                const float wy = ((q >> 1) & 1) ? c.fy : 1.0f - c.fy, sy = ((q >> 1) & 1) ? 1.0f : -1.0f;
+                const float wz = (q >> 2) ? c.fz : 1.0f - c.fz, sz = (q >> 2) ? 1.0f : -1.0f;
+                ax = fmaf(v, sx * wy * wz, ax);
+                ay = fmaf(v, wx * sy * wz, ay);
+                az = fmaf(v, wx * wy * sz, az);
+            }
+        }
+    }
+    gx[id] = ax * gv;
+    gy[id] = ay * gv;
+    gz[id] = az * gv;
+}
+
+__global__ void k_fill0(float* __restrict__ p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+int check_grid(const int grid[3])
+{
+    if (!grid) return NEFCT_ERR_NULL;
+    if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0) return NEFCT_ERR_SHAPE;
+    if (grid[1] > 65535 || grid[2] > 65535) return NEFCT_ERR_SHAPE;
+    const size_t n = (size_t)grid[0] * grid[1] * grid[2];
+    if (n * 4 >= 0x7fffffffull) return NEFCT_ERR_SHAPE;
+    return NEFCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nefct_deform(const float* image, const float* mx, const float* my, const float* mz, const int grid[3],
+                 float* deformed_image, int boundary, void* stream)
+{
+    int st = check_grid(grid);
+    if (st) return st;
+    if (!image || !mx || !my || !mz || !deformed_image) return NEFCT_ERR_NULL;
+    if (boundary != 0 && boundary != 1) return NEFCT_ERR_UNSUPPORTED;
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    hipLaunchKernelGGL(k_deform, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, (hipStream_t)stream, image, mx, my, mz,
+                       nx, ny, nz, boundary, deformed_image);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+int nefct_deform_adjoint(const float* grad_out, const float* mx, const float* my, const float* mz, const int grid[3],
+                         float* grad_image, int boundary, void* stream)
+{
+    int st = check_grid(grid);
+    if (st) return st;
+    if (!grad_out || !mx || !my || !mz || !grad_image) return NEFCT_ERR_NULL;
+    if (boundary != 0 && boundary != 1) return NEFCT_ERR_UNSUPPORTED;
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    const size_t n = (size_t)nx * ny * nz;
+    hipStream_t s = (hipStream_t)stream;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill0, dim3((unsigned)blocks), dim3(256), 0, s, grad_image, n);
+    hipLaunchKernelGGL(k_deform_adjoint, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, s, grad_out, mx, my, mz, nx,
+                       ny, nz, boundary, grad_image, (uint32_t)(n * 4));
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+int nefct_deform_dvf_grad(const float* image, const float* grad_out, const float* mx, const float* my,
+                          const float* mz, const int grid[3], float* grad_mx, float* grad_my, float* grad_mz,
+                          int boundary, void* stream)
+{
+    int st = check_grid(grid);
+    if (st) return st;
+    if (!image || !grad_out || !mx || !my || !mz || !grad_mx || !grad_my || !grad_mz) return NEFCT_ERR_NULL;
+    if (boundary != 0 && boundary != 1) return NEFCT_ERR_UNSUPPORTED;
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    hipLaunchKernelGGL(k_deform_dvf_grad, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, (hipStream_t)stream, image,
+                       grad_out, mx, my, mz, nx, ny, nz, boundary, grad_mx, grad_my, grad_mz);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+}  // extern "C"

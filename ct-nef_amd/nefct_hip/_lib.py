@@ -39,10 +39,18 @@ SIGNATURES = {
     "nefct_backproject_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_siddon_count": [_vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u, _vp, _sz, _vp],
+    "nefct_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "nefct_deform_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "nefct_deform_dvf_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "nefct_fdk_preweight": [_vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _f, _i, _f, _vp],
+    "nefct_fdk_filter": [_vp, _vp, _i, _i, _vp, _vp],
+    "nefct_fdk_workspace_size": [_i],
+    "nefct_fdk_backproject": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp, _sz, _vp],
 }
 _RESTYPE = {
     "nefct_status_string": ctypes.c_char_p,
     "nefct_siddon_workspace_size": ctypes.c_size_t,
+    "nefct_fdk_workspace_size": ctypes.c_size_t,
 }
 
 _lib = None

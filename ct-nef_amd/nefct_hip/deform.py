@@ -1,0 +1,115 @@
+"""Trilinear warp by a dense displacement field (the reference's ``Deform`` op)
+and its autograd Function.
+
+``deform(image, mx, my, mz)`` mirrors the op ``Deform(image, mx, my, mz, grid)``
+(cpp/deform/deform.cc:12-22): displacements in voxel units, volumes x fastest.
+``boundary=0`` reproduces the reference's rule (a voxel is written only when
+every base index lies in [1, n-2], cpp/deform/deform.cu.cc:31-33, else 0 as the
+zero-filled output of nefct/cc/deform/deform.cc:94); ``boundary=1`` is zero
+padding.  ``Warp`` differentiates with respect to the image (transpose
+scatter) and the displacement field.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .ops import _need_device, _ptr, _stream_ptr
+
+_vp = ctypes.c_void_p
+
+
+def _grid(t):
+    nz, ny, nx = t.shape[-3:]
+    return [int(nx), int(ny), int(nz)]
+
+
+@torch.library.custom_op("nefct::deform", mutates_args=())
+def deform_op(image: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor,
+              boundary: int) -> torch.Tensor:
+    _need_device(image, mx, my, mz)
+    for m in (mx, my, mz):
+        if m.shape != image.shape:
+            raise _lib.NefctError(f"displacement shape {tuple(m.shape)} != image shape {tuple(image.shape)}")
+    image, mx, my, mz = (t.contiguous() for t in (image, mx, my, mz))
+    out = torch.empty_like(image)
+    st = _lib.load().nefct_deform(_ptr(image), _ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(image)), _ptr(out),
+                                  int(boundary), _vp(_stream_ptr(image.device)))
+    _lib.check(st, "deform")
+    return out
+
+
+@deform_op.register_fake
+def _(image, mx, my, mz, boundary):
+    return torch.empty_like(image)
+
+
+@torch.library.custom_op("nefct::deform_adjoint", mutates_args=())
+def deform_adjoint_op(grad: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor,
+                      boundary: int) -> torch.Tensor:
+    _need_device(grad, mx, my, mz)
+    grad, mx, my, mz = (t.contiguous() for t in (grad, mx, my, mz))
+    out = torch.empty_like(grad)
+    st = _lib.load().nefct_deform_adjoint(_ptr(grad), _ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(grad)),
+                                          _ptr(out), int(boundary), _vp(_stream_ptr(grad.device)))
+    _lib.check(st, "deform_adjoint")
+    return out
+
+
+@deform_adjoint_op.register_fake
+def _(grad, mx, my, mz, boundary):
+    return torch.empty_like(grad)
+
+
+@torch.library.custom_op("nefct::deform_dvf_grad", mutates_args=())
+def deform_dvf_grad_op(image: torch.Tensor, grad: torch.Tensor, mx: torch.Tensor, my: torch.Tensor,
+                       mz: torch.Tensor, boundary: int) -> torch.Tensor:
+    """Returns [3, *shape]: d<grad, deform(image, m)>/d(mx, my, mz)."""
+    _need_device(image, grad, mx, my, mz)
+    image, grad, mx, my, mz = (t.contiguous() for t in (image, grad, mx, my, mz))
+    out = torch.empty((3,) + tuple(image.shape), dtype=torch.float32, device=image.device)
+    st = _lib.load().nefct_deform_dvf_grad(_ptr(image), _ptr(grad), _ptr(mx), _ptr(my), _ptr(mz),
+                                           _lib.int3(_grid(image)), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                           int(boundary), _vp(_stream_ptr(image.device)))
+    _lib.check(st, "deform_dvf_grad")
+    return out
+
+
+@deform_dvf_grad_op.register_fake
+def _(image, grad, mx, my, mz, boundary):
+    return image.new_empty((3,) + tuple(image.shape))
+
+
+def deform(image, mx, my, mz, boundary: int = 0):
+    """Warped volume (no autograd)."""
+    return deform_op(image, mx, my, mz, int(boundary))
+
+
+class Warp(torch.autograd.Function):
+    """out = image(i + dvf(i)) with dvf [3, nz, ny, nx] = (mx, my, mz) in voxel
+    units.  Backward: image gradient by the transpose scatter kernel,
+    displacement gradient from the trilinear weight derivatives.  Only the
+    inputs are kept for backward; the warped volume is never stored."""
+
+    @staticmethod
+    def forward(ctx, image, dvf, boundary=1):
+        ctx.boundary = int(boundary)
+        ctx.save_for_backward(image, dvf)
+        return deform_op(image, dvf[0], dvf[1], dvf[2], int(boundary))
+
+    @staticmethod
+    def backward(ctx, g):
+        image, dvf = ctx.saved_tensors
+        g = g.contiguous()
+        gi = gd = None
+        if ctx.needs_input_grad[0]:
+            gi = deform_adjoint_op(g, dvf[0], dvf[1], dvf[2], ctx.boundary)
+        if ctx.needs_input_grad[1]:
+            gd = deform_dvf_grad_op(image, g, dvf[0], dvf[1], dvf[2], ctx.boundary)
+        return gi, gd, None
+
+
+def warp(image, dvf, boundary: int = 1):
+    return Warp.apply(image, dvf, boundary)

@@ -111,6 +111,64 @@ int nefct_siddon_count(const int grid[3], const float center[3], const float siz
                        int nb, float db, float bi, int cyl, int32_t* counts, unsigned flags,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Trilinear warp                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Pull-trilinear warp: out[i] = image(i + (mx,my,mz)[i]), displacement in
+ * voxel units, all volumes x fastest with the same grid.  boundary 0
+ * reproduces the reference: a voxel is written only when every base index
+ * floor(p) is in [1, n-2] (cpp/deform/deform.cu.cc:31-33), else 0 (the output
+ * is zero-filled as in nefct/cc/deform/deform.cc:94); boundary 1 = zero
+ * padding (each in-range corner contributes).  Replaces DeformOp::Compute
+ * (cpp/deform/deform.cc:64-90) + deform (cpp/deform/deform.cu.cc:112-126). */
+int nefct_deform(const float* image, const float* mx, const float* my, const float* mz, const int grid[3],
+                 float* deformed_image, int boundary, void* stream);
+
+/* Transpose of nefct_deform in the image (scatter of grad_out to the 8
+ * corners).  grad_image is fully written.  No reference counterpart (the
+ * reference registers no gradient); it is the backward of the warp. */
+int nefct_deform_adjoint(const float* grad_out, const float* mx, const float* my, const float* mz,
+                         const int grid[3], float* grad_image, int boundary, void* stream);
+
+/* Gradient of <grad_out, deform(image, m)> with respect to mx, my, mz
+ * (voxel units).  No reference counterpart; backward of the warp. */
+int nefct_deform_dvf_grad(const float* image, const float* grad_out, const float* mx, const float* my,
+                          const float* mz, const int grid[3], float* grad_mx, float* grad_my, float* grad_mz,
+                          int boundary, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* FDK (flat detector)                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* out = projection_value * cosine weight * [Parker weight] * scale.
+ * Cosine weight SAD/sqrt(SAD^2 + a^2 + b^2) (mat_codes/filtering.m:5-14),
+ * Parker short-scan weights with views at ((180/dang - nv)/2 + i + 1) dang
+ * degrees (mat_codes/ParkerWeight.m:5-27); direction +1/-1 is the rotation
+ * sense.  No reference CUDA counterpart (the reference does this in MATLAB). */
+int nefct_fdk_preweight(const float* projection_value, float* out, int nv, float SID, float SAD, int na, float da,
+                        float ai, int nb, float db, float bi, int parker, float dang_deg, int direction, float scale,
+                        void* stream);
+
+/* Row filtering: out[r][u] = sum_v in[r][v] taps[u - v + na - 1] for rows r
+ * of length na (<= 2048), taps of length 2 na - 1 (the spatial form of the
+ * windowed ramp of mat_codes/filtering.m:20-41,54-88; in != out). */
+int nefct_fdk_filter(const float* in, float* out, int rows, int na, const float* taps, void* stream);
+
+/* Workspace bytes of nefct_fdk_backproject (per-view table). */
+size_t nefct_fdk_workspace_size(int nv);
+
+/* Voxel-driven FDK backprojection: image[v] = sum over views of
+ * (SID/U)^2 * bilinear(filtered view at the projection of voxel centre v),
+ * U = SID + x cos t + y sin t; offsets[iv] (may be NULL) shifts source and
+ * detector along z (helical), as dist_3d_back_fdk.cu.cc:44-65.  Image fully
+ * written.  Replaces back_project_flat_fdk_gpu (dist_3d_back_fdk.cu.cc:257-276)
+ * with the voxel-driven sampling of mat_codes/backprojection.m:11-25. */
+int nefct_fdk_backproject(const float* filtered, const int grid[3], const float center[3], const float size[3],
+                          const float* angles, const float* offsets, int nv, float SID, float SAD, int na, float da,
+                          float ai, int nb, float db, float bi, float* image, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -239,237 +239,43 @@ long orc_exact_count(const orc_vol *V, const orc_det *G, const double *angles, i
 }
 
 /* ------------------------------------------------------------------ */
-/* (2) FP32 restatement of the reference slab walk                     */
+/* (2) restatement of the reference slab walk, FP32 and FP64           */
 /* ------------------------------------------------------------------ */
-typedef struct {
-    int nx, ny, nz;
-    float cx, cy, cz, sx, sy, sz;
-} vol32;
+#define REAL float
+#define SFX(n) n##_f32
+#define FLOOR floorf
+#define SQRT sqrtf
+#define FABS fabsf
+#define COS cosf
+#define SIN sinf
+#include "ref_walk.h"
+#undef REAL
+#undef SFX
+#undef FLOOR
+#undef SQRT
+#undef FABS
+#undef COS
+#undef SIN
+#define REAL double
+#define SFX(n) n##_f64
+#define FLOOR floor
+#define SQRT sqrt
+#define FABS fabs
+#define COS cos
+#define SIN sin
+#include "ref_walk.h"
 
-/* Reference ray set-up in float (siddon.cu.cc:341-353 / :376-387). */
-static void ref_points32(const orc_det *g, float angle, int ia, int ib, float S[3], float D[3])
-{
-    const float na = (float)g->na, nb = (float)g->nb;
-    const float da = (float)g->da, db = (float)g->db, ai = (float)g->ai, bi = (float)g->bi;
-    const float SID = (float)g->sid, SAD = (float)g->sad;
-    const float a = (ai + ia) * da - na * da / 2;
-    const float b = (bi + ib) * db - nb * db / 2;
-    const float c = cosf(angle), s = sinf(angle);
-    float lx, ly;
-    if (g->cyl) {
-        lx = SAD * cosf(a) - SID;
-        ly = SAD * sinf(a);
-    } else {
-        lx = SAD - SID;
-        ly = a;
-    }
-    S[0] = -SID * c;
-    S[1] = -SID * s;
-    S[2] = 0.0f;
-    D[0] = lx * c - ly * s;
-    D[1] = lx * s + ly * c;
-    D[2] = b;
-}
-
-/* Visitor for one weighted voxel: forward accumulates, adjoint scatters. */
-typedef struct {
-    const float *img; /* forward */
-    float *out;       /* adjoint */
-    float val;        /* adjoint ray value */
-    float acc;        /* forward accumulator */
-    int fwd;
-} ref_acc;
-
-static inline void ref_visit(ref_acc *A, long idx, float w)
-{
-    if (A->fwd) A->acc += A->img[idx] * w;
-    else A->out[idx] += A->val * w;
-}
-
-/*
- * One slab of the reference walk.  'lo'/'hi' are the sorted transverse
- * coordinates (index units) at the two slab faces, as the reference computes
- * yy1<=yy2 and zz1<=zz2 (siddon.cu.cc:52-76).  'flip_z' is true when the z
- * ordering is measured from the opposite slab face to the y ordering
- * (sign(ky) != sign(kz)); the reference ignores it (Q2), the corrected rule
- * uses it.
- */
-static void ref_slab(ref_acc *A, float p1, float p2, float q1, float q2, int np, int nq,
-                     long base, long sp, long sq, float wbase, int ref_slab_rule, int flip_q)
-{
-    const int cp1 = (int)floorf(p1), cp2 = (int)floorf(p2);
-    const int cq1 = (int)floorf(q1), cq2 = (int)floorf(q2);
-#define VOX(ip, iq) (base + (long)(ip) * sp + (long)(iq) * sq)
-    if (cp1 == cp2) {
-        if (!(0 <= cp1 && cp1 < np)) return;
-        if (cq1 == cq2) {
-            if (0 <= cq1 && cq1 < nq) ref_visit(A, VOX(cp1, cq1), wbase);
-        } else if (-1 <= cq1 && cq1 < nq) {
-            const float rq = (cq2 - q1) / (q2 - q1);
-            if (cq1 >= 0) ref_visit(A, VOX(cp1, cq1), rq * wbase);
-            if (cq2 < nq) ref_visit(A, VOX(cp1, cq2), (1 - rq) * wbase);
-        }
-        return;
-    }
-    if (!(-1 <= cp1 && cp1 < np)) return;
-    if (cq1 == cq2) {
-        if (0 <= cq1 && cq1 < nq) {
-            const float rp = (cp2 - p1) / (p2 - p1);
-            if (cp1 >= 0) ref_visit(A, VOX(cp1, cq1), rp * wbase);
-            if (cp2 < np) ref_visit(A, VOX(cp2, cq1), (1 - rp) * wbase);
-        }
-        return;
-    }
-    if (!(-1 <= cq1 && cq1 < nq)) return;
-    const float rp = (cp2 - p1) / (p2 - p1);
-    const float rq = (cq2 - q1) / (q2 - q1);
-    if (ref_slab_rule || !flip_q) {
-        /* reference attribution: both fractions from the low end (siddon.cu.cc:132-176) */
-        if (rp > rq) {
-            if (cp1 >= 0 && cq1 >= 0) ref_visit(A, VOX(cp1, cq1), rq * wbase);
-            if (cp1 >= 0 && cq2 < nq) ref_visit(A, VOX(cp1, cq2), (rp - rq) * wbase);
-            if (cp2 < np && cq2 < nq) ref_visit(A, VOX(cp2, cq2), (1 - rp) * wbase);
-        } else {
-            if (cp1 >= 0 && cq1 >= 0) ref_visit(A, VOX(cp1, cq1), rp * wbase);
-            if (cp2 < np && cq1 >= 0) ref_visit(A, VOX(cp2, cq1), (rq - rp) * wbase);
-            if (cp2 < np && cq2 < nq) ref_visit(A, VOX(cp2, cq2), (1 - rq) * wbase);
-        }
-    } else {
-        /* corrected: walk from the p-low face; there q starts in cq2 and
-         * crosses down into cq1 at fraction (1 - rq). */
-        const float fq = 1 - rq;
-        if (rp < fq) {
-            if (cp1 >= 0 && cq2 < nq) ref_visit(A, VOX(cp1, cq2), rp * wbase);
-            if (cp2 < np && cq2 < nq) ref_visit(A, VOX(cp2, cq2), (fq - rp) * wbase);
-            if (cp2 < np && cq1 >= 0) ref_visit(A, VOX(cp2, cq1), (1 - fq) * wbase);
-        } else {
-            if (cp1 >= 0 && cq2 < nq) ref_visit(A, VOX(cp1, cq2), fq * wbase);
-            if (cp1 >= 0 && cq1 >= 0) ref_visit(A, VOX(cp1, cq1), (rp - fq) * wbase);
-            if (cp2 < np && cq1 >= 0) ref_visit(A, VOX(cp2, cq1), (1 - rp) * wbase);
-        }
-    }
-#undef VOX
-}
-
-/* One ray of the reference walk (project_device / backproject_device). */
-static void ref_ray(const vol32 *V, const float S[3], const float D[3], unsigned flags, int adjoint, ref_acc *A)
-{
-    const int nx = V->nx, ny = V->ny, nz = V->nz;
-    const float dx_ = V->sx / nx;
-    const float dy = V->sy / ny / dx_;
-    const float dz = V->sz / nz / dx_;
-    const float x1 = (S[0] - V->cx) / dx_, x2 = (D[0] - V->cx) / dx_;
-    const float y1 = (S[1] - V->cy) / dx_, y2 = (D[1] - V->cy) / dx_;
-    const float z1 = (S[2] - V->cz) / dx_, z2 = (D[2] - V->cz) / dx_;
-    const float xd = x2 - x1, yd = y2 - y1, zd = z2 - z1;
-    if (adjoint && (flags & ORC_SHORT_SKIP) && sqrtf(xd * xd + yd * yd) < 10.0f) return;
-    const float nx2 = nx / 2.0f, ny2 = ny / 2.0f, nz2 = nz / 2.0f;
-    const float L = sqrtf(xd * xd + yd * yd + zd * zd);
-    const int refrule = (flags & ORC_REF_SLAB) != 0;
-    const int xdom = (adjoint && (flags & ORC_TIE_GT)) ? (fabsf(xd) > fabsf(yd)) : (fabsf(xd) >= fabsf(yd));
-    if (xdom) {
-        const float ky = yd / xd, kz = zd / xd;
-        float wbase = sqrtf(1 + ky * ky + kz * kz) * dx_;
-        if (flags & ORC_INV_L2) wbase = wbase / L / L;
-        const int flip = (ky >= 0.0f) != (kz >= 0.0f);
-        for (int ix = 0; ix < nx; ++ix) {
-            const float xx1 = ix - nx2, xx2 = xx1 + 1.0f;
-            float yy1, yy2, zz1, zz2;
-            if (ky >= 0.0f) { yy1 = (y1 + ky * (xx1 - x1)) / dy + ny2; yy2 = (y1 + ky * (xx2 - x1)) / dy + ny2; }
-            else { yy1 = (y1 + ky * (xx2 - x1)) / dy + ny2; yy2 = (y1 + ky * (xx1 - x1)) / dy + ny2; }
-            if (kz >= 0.0f) { zz1 = (z1 + kz * (xx1 - x1)) / dz + nz2; zz2 = (z1 + kz * (xx2 - x1)) / dz + nz2; }
-            else { zz1 = (z1 + kz * (xx2 - x1)) / dz + nz2; zz2 = (z1 + kz * (xx1 - x1)) / dz + nz2; }
-            ref_slab(A, yy1, yy2, zz1, zz2, ny, nz, ix, nx, (long)nx * ny, wbase, refrule, flip);
-        }
-    } else {
-        const float kx = xd / yd, kz = zd / yd;
-        float wbase = sqrtf(1 + kx * kx + kz * kz) * dx_;
-        if (flags & ORC_INV_L2) wbase = wbase / L / L;
-        const int flip = (kx >= 0.0f) != (kz >= 0.0f);
-        for (int iy = 0; iy < ny; ++iy) {
-            const float yy1 = iy - ny2, yy2 = yy1 + 1.0f;
-            float xx1, xx2, zz1, zz2;
-            if (kx >= 0.0f) { xx1 = (x1 + kx * (yy1 - y1)) + nx2; xx2 = (x1 + kx * (yy2 - y1)) + nx2; }
-            else { xx1 = (x1 + kx * (yy2 - y1)) + nx2; xx2 = (x1 + kx * (yy1 - y1)) + nx2; }
-            if (kz >= 0.0f) { zz1 = (z1 + kz * (yy1 - y1)) / dz + nz2; zz2 = (z1 + kz * (yy2 - y1)) / dz + nz2; }
-            else { zz1 = (z1 + kz * (yy2 - y1)) / dz + nz2; zz2 = (z1 + kz * (yy1 - y1)) / dz + nz2; }
-            ref_slab(A, xx1, xx2, zz1, zz2, nx, nz, (long)iy * nx, 1, (long)nx * ny, wbase, refrule, flip);
-        }
-    }
-}
-
-static void to_vol32(const orc_vol *V, vol32 *W)
-{
-    W->nx = V->nx; W->ny = V->ny; W->nz = V->nz;
-    W->cx = (float)V->cx; W->cy = (float)V->cy; W->cz = (float)V->cz;
-    W->sx = (float)V->sx; W->sy = (float)V->sy; W->sz = (float)V->sz;
-}
-
-/* Reference forward over views [v0, v0 + step*k) (all views: v0=0, step=1). */
+/* the FP32 entry points keep their plain names */
 void orc_ref_fwd(const orc_vol *V, const orc_det *G, const float *angles, int nv,
                  const float *img, float *out, unsigned flags, int v0, int vstep)
 {
-    vol32 W;
-    to_vol32(V, &W);
-    if (vstep <= 0) vstep = 1;
-    const int nsel = (nv - v0 + vstep - 1) / vstep;
-    const long per = (long)G->na * G->nb;
-    const long nrays = per * nsel;
-#pragma omp parallel for schedule(dynamic, 256)
-    for (long r = 0; r < nrays; ++r) {
-        const int ia = (int)(r % G->na), ib = (int)((r / G->na) % G->nb);
-        const int iv = v0 + vstep * (int)(r / per);
-        float S[3], D[3];
-        ref_points32(G, angles[iv], ia, ib, S, D);
-        ref_acc A = {img, NULL, 0.f, 0.f, 1};
-        ref_ray(&W, S, D, flags, 0, &A);
-        out[ia + (long)G->na * (ib + (long)G->nb * iv)] = A.acc;
-    }
+    orc_ref_fwd_f32(V, G, angles, nv, img, out, flags, v0, vstep);
 }
 
-/* Reference adjoint; out is zeroed here (siddon.cc:265).  Threads own
- * private volumes over disjoint view subsets, then reduce. */
 void orc_ref_bp(const orc_vol *V, const orc_det *G, const float *angles, int nv,
                 const float *pv, float *out, unsigned flags, int v0, int vstep)
 {
-    vol32 W;
-    to_vol32(V, &W);
-    if (vstep <= 0) vstep = 1;
-    const long nvox = (long)V->nx * V->ny * V->nz;
-    memset(out, 0, sizeof(float) * nvox);
-    int nthr = 1;
-#ifdef _OPENMP
-    nthr = omp_get_max_threads();
-#endif
-    float *priv = (float *)calloc((size_t)nthr * nvox, sizeof(float));
-    if (!priv) return;
-#pragma omp parallel
-    {
-        int t = 0;
-#ifdef _OPENMP
-        t = omp_get_thread_num();
-#endif
-        float *mine = priv + (size_t)t * nvox;
-        const int nsel = (nv - v0 + vstep - 1) / vstep;
-        const long rows = (long)nsel * G->nb;
-#pragma omp for schedule(dynamic, 4)
-        for (long row = 0; row < rows; ++row) {
-            const int iv = v0 + vstep * (int)(row / G->nb), ib = (int)(row % G->nb);
-                for (int ia = 0; ia < G->na; ++ia) {
-                    float S[3], D[3];
-                    ref_points32(G, angles[iv], ia, ib, S, D);
-                    ref_acc A = {NULL, mine, pv[ia + (long)G->na * (ib + (long)G->nb * iv)], 0.f, 0};
-                    ref_ray(&W, S, D, flags, 1, &A);
-                }
-        }
-    }
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < nvox; ++i) {
-        float acc = 0.0f;
-        for (int t = 0; t < nthr; ++t) acc += priv[(size_t)t * nvox + i];
-        out[i] = acc;
-    }
-    free(priv);
+    orc_ref_bp_f32(V, G, angles, nv, pv, out, flags, v0, vstep);
 }
 
 /* Threads the oracle would use (for the baseline record). */

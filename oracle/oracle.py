@@ -90,6 +90,8 @@ def lib():
         L.orc_ray_points.argtypes = [P(_Det), d, i, i, P(d), P(d)]
         L.orc_ref_fwd.argtypes = [P(_Vol), P(_Det), P(f), i, P(f), P(f), ctypes.c_uint, i, i]
         L.orc_ref_bp.argtypes = [P(_Vol), P(_Det), P(f), i, P(f), P(f), ctypes.c_uint, i, i]
+        L.orc_ref_fwd_f64.argtypes = [P(_Vol), P(_Det), P(d), i, P(d), P(d), ctypes.c_uint, i, i]
+        L.orc_ref_bp_f64.argtypes = [P(_Vol), P(_Det), P(d), i, P(d), P(d), ctypes.c_uint, i, i]
         L.orc_threads.restype = i
         _lib = L
     return _lib
@@ -194,6 +196,28 @@ def ref_bp(vol, det, angles, pv, flags=REF_BP_FLAGS, v0=0, vstep=1):
     return out
 
 
+def ref_fwd64(vol, det, angles, img_zyx, flags=REF_FWD_FLAGS):
+    """The reference slab rule evaluated in FP64 (same restatement, no FP32
+    rounding): the target for the reference-mode GPU kernels, which follow the
+    rule with their own FP32 arithmetic.  Angles are taken as given (pass the
+    float32 angles the GPU sees, widened)."""
+    img = np.ascontiguousarray(img_zyx, dtype=np.float64)
+    ang = np.ascontiguousarray(angles, dtype=np.float64)
+    out = np.zeros((len(ang), det.nb, det.na), np.float64)
+    lib().orc_ref_fwd_f64(ctypes.byref(vol), ctypes.byref(det), _ptr(ang, ctypes.c_double), len(ang),
+                          _ptr(img, ctypes.c_double), _ptr(out, ctypes.c_double), flags, 0, 1)
+    return out
+
+
+def ref_bp64(vol, det, angles, pv, flags=REF_BP_FLAGS):
+    p = np.ascontiguousarray(pv, dtype=np.float64)
+    ang = np.ascontiguousarray(angles, dtype=np.float64)
+    out = np.zeros((vol.nz, vol.ny, vol.nx), np.float64)
+    lib().orc_ref_bp_f64(ctypes.byref(vol), ctypes.byref(det), _ptr(ang, ctypes.c_double), len(ang),
+                         _ptr(p, ctypes.c_double), _ptr(out, ctypes.c_double), flags, 0, 1)
+    return out
+
+
 # --------------------------------------------------------------------------
 # Trilinear warp (cpp/deform/deform.cu.cc:13-46)
 # --------------------------------------------------------------------------
@@ -207,9 +231,11 @@ def _warp_corners(shape_zyx, mx, my, mz, boundary):
     """
     nz, ny, nx = shape_zyx
     iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    px = ix + mx.astype(np.float64)
-    py = iy + my.astype(np.float64)
-    pz = iz + mz.astype(np.float64)
+    # the sample position is formed in float32 as the reference does
+    # (deform.cu.cc:25-36: floor(ix + mx[id]) and ix + mx[id] - xi); weights in FP64
+    px = (ix.astype(np.float32) + np.asarray(mx, np.float32)).astype(np.float64)
+    py = (iy.astype(np.float32) + np.asarray(my, np.float32)).astype(np.float64)
+    pz = (iz.astype(np.float32) + np.asarray(mz, np.float32)).astype(np.float64)
     x0 = np.floor(px).astype(np.int64)
     y0 = np.floor(py).astype(np.int64)
     z0 = np.floor(pz).astype(np.int64)
@@ -298,12 +324,20 @@ def parker_weights(nv, dang_rad, det, direction=1):
     w = np.ones((nv, det.na))
     for i in range(nv):
         p = ((n180 - nv) / 2.0 + (i + 1)) / n180 * 180.0 + direction * as_deg
-        r1 = (p <= fan / 2) & (p > -fan / 2)
-        w[i, r1] = np.maximum(np.sin(np.pi / 2 * (fan / 2 + p[r1]) / fan), 0) ** 2
-        r2 = (p >= 180 - fan / 2) & (p < 180 + fan / 2)
-        w[i, r2] = np.maximum(np.cos(np.pi / 2 * (fan / 2 + p[r2] - 180) / fan), 0) ** 2
-        r3 = (p < -fan / 2) | (p > 180 + fan / 2)
-        w[i, r3] = 0.0
+        w[i] = parker_w(p, fan)
+    return w
+
+
+def parker_w(p, fan):
+    """Weight as a function of p = beta + gamma (degrees), ParkerWeight.m:11-24."""
+    p = np.asarray(p, np.float64)
+    w = np.ones_like(p)
+    r1 = (p <= fan / 2) & (p > -fan / 2)
+    w[r1] = np.maximum(np.sin(np.pi / 2 * (fan / 2 + p[r1]) / fan), 0) ** 2
+    r2 = (p >= 180 - fan / 2) & (p < 180 + fan / 2)
+    w[r2] = np.maximum(np.cos(np.pi / 2 * (fan / 2 + p[r2] - 180) / fan), 0) ** 2
+    r3 = (p < -fan / 2) | (p > 180 + fan / 2)
+    w[r3] = 0.0
     return w
 
 

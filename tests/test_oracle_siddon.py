@@ -281,3 +281,38 @@ def test_exact_count_matches_per_ray(oracle):
     assert total == int(per.sum()) and total > 0
     S, D = oracle.ray_points(det, 0.9, 3, 2)
     assert per[1, 2, 3] == oracle.exact_ray_count(V, S, D)[0]
+
+
+@pytest.mark.parametrize("cyl", [0, 1])
+def test_ref_rule_fp64_corrected_equals_exact(oracle, cyl):
+    """The FP64 instantiation of the reference restatement, with Q2 corrected and
+    no 1/L^2, is the exact Siddon transform: it must equal the merged-alpha
+    oracle to rounding (1e-11), forward and adjoint."""
+    rng = np.random.default_rng(11)
+    shape = (20, 20, 16)
+    V = oracle.vol_struct(shape, (0.3, -0.2, 0.1), (20.0, 20.0, 16 * 0.8))
+    det = (oracle.det_struct(1, 60.0, 100.0, 24, 0.012, 0.5, 20, 1.2, 0.5) if cyl
+           else oracle.det_struct(0, 60.0, 100.0, 24, 1.3, 0.5, 20, 1.2, 0.5))
+    ang = np.linspace(0, 2 * np.pi, 11, endpoint=False) + 0.05
+    img = rng.uniform(size=shape[::-1])
+    a = oracle.ref_fwd64(V, det, ang, img, flags=0)
+    b = oracle.exact_fwd(V, det, ang, img)
+    assert np.linalg.norm(a - b) <= 1e-11 * np.linalg.norm(b)
+    y = rng.uniform(size=a.shape)
+    c = oracle.ref_bp64(V, det, ang, y, flags=0)
+    d = oracle.exact_bp(V, det, ang, y)
+    assert np.linalg.norm(c - d) <= 1e-11 * np.linalg.norm(d)
+
+
+def test_reference_fp32_noise_floor(oracle):
+    """Documents why reference-mode parity targets the FP64 rule: the FP32
+    restatement itself differs from the same rule in FP64 by up to ~1e-5."""
+    shape = (48, 40, 36)
+    V = oracle.vol_struct(shape, (3.0, -2.0, 1.5), (48 * 1.1, 40 * 1.1, 36 * 0.9))
+    det = oracle.det_struct(0, 300.0, 500.0, 70, 2.3, 0.3, 45, 2.1, 0.6)
+    ang = (2 * np.pi * np.arange(16) / 16).astype(np.float32) + np.float32(0.013)
+    y = np.random.default_rng(1).uniform(0, 1, size=(16, 45, 70)).astype(np.float32)
+    c = oracle.ref_bp(V, det, ang, y).astype(np.float64)
+    d = oracle.ref_bp64(V, det, ang.astype(np.float64), y)
+    rel = np.linalg.norm(c - d) / np.linalg.norm(d)
+    assert 1e-7 < rel < 5e-5
