@@ -19,6 +19,7 @@
 //    copy, so an index error reads 0 / drops the atomic instead of faulting;
 //  * the adjoint scatters with buffer float atomics into the same two
 //    layouts, then one pass sums them into the x-fastest output.
+#include <algorithm>
 #include <cmath>
 #include <initializer_list>
 #include <math.h>
@@ -37,6 +38,7 @@ struct VolDev {
     int n[3];
     double lo[3];
     double vs[3];
+    double inv_vs[3];
 };
 
 struct DetDev {
@@ -95,28 +97,31 @@ __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, 
     D[2] = b;
 }
 
-// Restricts [tin, tout] to the slab parameters where lo <= A + k t <= hi.
-__device__ __forceinline__ void clip_axis(double k, double A, double lo, double hi, double& tin, double& tout)
+// Restricts [tin, tout] to the slab parameters where lo <= A + k t <= hi
+// (inv_k = 1/k, unused when k == 0).
+__device__ __forceinline__ void clip_axis(double k, double inv_k, double A, double lo, double hi, double& tin,
+                                          double& tout)
 {
     if (k == 0.0) {
         if (!(A > lo && A < hi)) { tin = 1.0; tout = 0.0; }
         return;
     }
-    double t0 = (lo - A) / k, t1 = (hi - A) / k;
+    double t0 = (lo - A) * inv_k, t1 = (hi - A) * inv_k;
     if (t0 > t1) { double t = t0; t0 = t1; t1 = t; }
     tin = fmax(tin, t0);
     tout = fmin(tout, t1);
 }
 
-// FP64 ray set-up.  'ref' selects the reference's x/y-only dominance rule.
+// FP64 ray set-up (three divisions and one square root per ray).
+// 'ref' selects the reference's x/y-only dominance rule.
 __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
                           unsigned flags, bool adjoint, Ray& r)
 {
     double S[3], D[3], uS[3], d[3];
     ray_points(G, c, s, ia, ib, S, D);
     for (int i = 0; i < 3; ++i) {
-        uS[i] = (S[i] - V.lo[i]) / V.vs[i];
-        d[i] = (D[i] - S[i]) / V.vs[i];
+        uS[i] = (S[i] - V.lo[i]) * V.inv_vs[i];
+        d[i] = (D[i] - S[i]) * V.inv_vs[i];
     }
     r.dom = -1;
     r.s0 = r.s1 = 0;
@@ -126,7 +131,7 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     int m;
     const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
     // reference quantities in x-voxel units (siddon.cu.cc:19-38)
-    const double xd = (D[0] - S[0]) / V.vs[0], yd = (D[1] - S[1]) / V.vs[0], zd = (D[2] - S[2]) / V.vs[0];
+    const double xd = (D[0] - S[0]) * V.inv_vs[0], yd = (D[1] - S[1]) * V.inv_vs[0], zd = (D[2] - S[2]) * V.inv_vs[0];
     if (ref) {
         if (adjoint && (flags & NEFCT_F_SHORT_SKIP) && sqrt(xd * xd + yd * yd) < 10.0) return;
         const bool xdom = (adjoint && (flags & NEFCT_F_TIE_GT)) ? (fabs(xd) > fabs(yd)) : (fabs(xd) >= fabs(yd));
@@ -138,7 +143,10 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     if (d[m] == 0.0) return;
     const int a = (m == 0) ? 1 : 0;
     const int b = (m == 2) ? 1 : 2;
-    const double ka = d[a] / d[m], kb = d[b] / d[m];
+    const double inv_dm = 1.0 / d[m];
+    const double ka = d[a] * inv_dm, kb = d[b] * inv_dm;
+    const double ika = (d[a] != 0.0) ? d[m] / d[a] : 0.0;
+    const double ikb = (d[b] != 0.0) ? d[m] / d[b] : 0.0;
     const double Aa = uS[a] - ka * uS[m], Ab = uS[b] - kb * uS[m];
     // The exact walk clips to the volume box.  The reference rule can credit an
     // in-range voxel from a slab whose exact segment lies just outside the box
@@ -147,8 +155,8 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     // then decide exactly as siddon.cu.cc:80-176 does.
     const double mg = ref ? 1.0 : 0.0;
     double tin = 0.0, tout = (double)V.n[m];
-    clip_axis(ka, Aa, -mg, V.n[a] + mg, tin, tout);
-    clip_axis(kb, Ab, -mg, V.n[b] + mg, tin, tout);
+    clip_axis(ka, ika, Aa, -mg, V.n[a] + mg, tin, tout);
+    clip_axis(kb, ikb, Ab, -mg, V.n[b] + mg, tin, tout);
     if (!(tout > tin)) return;
     int s0 = (int)floor(tin), s1 = (int)ceil(tout);
     s0 = max(s0, 0);
@@ -164,16 +172,13 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     r.kb = (float)kb;
     r.Ac = (float)(Aa + ka * tc - ha);
     r.Bc = (float)(Ab + kb * tc - hb);
-    r.inv_ka = (ka != 0.0) ? (float)(1.0 / ka) : 0.0f;
-    r.inv_kb = (kb != 0.0) ? (float)(1.0 / kb) : 0.0f;
+    r.inv_ka = (float)ika;
+    r.inv_kb = (float)ikb;
     r.pa = ka > 0.0 ? 1.0f : 0.0f;
     r.pb = kb > 0.0 ? 1.0f : 0.0f;
-    const double qa = ka * V.vs[a] / V.vs[m], qb = kb * V.vs[b] / V.vs[m];
+    const double qa = ka * V.vs[a] * V.inv_vs[m], qb = kb * V.vs[b] * V.inv_vs[m];
     double sc = V.vs[m] * sqrt(1.0 + qa * qa + qb * qb);
-    if (flags & NEFCT_F_INV_L2) {
-        const double L2 = xd * xd + yd * yd + zd * zd;
-        sc = sc / L2;
-    }
+    if (flags & NEFCT_F_INV_L2) sc = sc / (xd * xd + yd * yd + zd * zd);
     r.scale = (float)sc;
 }
 
@@ -231,7 +236,7 @@ __device__ __forceinline__ void slab_segments(float a0, float a1, float fla0, fl
         const float fa = (ca1 != ca0) ? clamp01((fla0 + r.pa - a0) * r.inv_ka) : 1.0f;
         const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - b0) * r.inv_kb) : 1.0f;
         const bool afirst = fa < fb;
-        const float f1 = fminf(fa, fb), f2 = fmaxf(fa, fb);
+        const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
         s.a1 = ca0; s.b1 = cb0;
         s.a2 = afirst ? ca1 : ca0;
         s.b2 = afirst ? cb0 : cb1;
@@ -265,6 +270,10 @@ __device__ __forceinline__ bool cell_in(const Walk& w, int ca, int cb)
 
 // MODE 0: forward (returns the weighted sum), 1: adjoint scatter of 'val',
 // 2: count of voxels with nonzero length.
+//
+// The three gathers of a slab are issued as three predicated loads whose
+// results are consumed only after all of them, so a wavefront has all of a
+// slab's loads (two slabs' with the unroll) in flight at once.
 template <int MODE, bool REF, bool CHECKED>
 __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Walk& w,
                           __amdgpu_buffer_rsrc_t rs, float val)
@@ -274,6 +283,8 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
     float a0 = fmaf(r.ka, tf, r.Ac), b0 = fmaf(r.kb, tf, r.Bc);
     float fla0 = floorf(a0), flb0 = floorf(b0);
     const float sv = val * r.scale;
+    int tb = tmin * w.st + w.org;
+#pragma unroll 2
     for (int t = tmin; t < tmax; ++t) {
         tf += 1.0f;
         const float a1 = fmaf(r.ka, tf, r.Ac), b1 = fmaf(r.kb, tf, r.Bc);
@@ -281,10 +292,9 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
         Seg s;
         slab_segments<REF>(a0, a1, fla0, fla1, b0, b1, flb0, flb1, r, w, s);
         const bool act = mine && (t >= r.s0) && (t < r.s1) && s.ok;
-        const int tb = t * w.st + w.org;
-        const int o1 = (s.a1 * w.sa + s.b1 * w.sb + tb) * 4;
-        const int o2 = (s.a2 * w.sa + s.b2 * w.sb + tb) * 4;
-        const int o3 = (s.a3 * w.sa + s.b3 * w.sb + tb) * 4;
+        const int o1 = (s.a1 + __mul24(s.b1, w.sb) + tb) * 4;
+        const int o2 = (s.a2 + __mul24(s.b2, w.sb) + tb) * 4;
+        const int o3 = (s.a3 + __mul24(s.b3, w.sb) + tb) * 4;
         bool u1 = act && s.w1 > 0.0f, u2 = act && s.w2 > 0.0f, u3 = act && s.w3 > 0.0f;
         if (CHECKED || REF || MODE == 2) {
             u1 = u1 && cell_in(w, s.a1, s.b1);
@@ -292,11 +302,13 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
             u3 = u3 && cell_in(w, s.a3, s.b3);
         }
         if (MODE == 0) {
-            float v = 0.0f;
-            if (u1) v = s.w1 * buf_load(rs, o1);
-            if (u2) v = fmaf(s.w2, buf_load(rs, o2), v);
-            if (u3) v = fmaf(s.w3, buf_load(rs, o3), v);
-            acc += v;
+            float v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
+            if (u1) v1 = buf_load(rs, o1);
+            if (u2) v2 = buf_load(rs, o2);
+            if (u3) v3 = buf_load(rs, o3);
+            acc = fmaf(s.w1, v1, acc);
+            acc = fmaf(s.w2, v2, acc);
+            acc = fmaf(s.w3, v3, acc);
         } else if (MODE == 1) {
             if (u1) buf_atomic_add(rs, o1, s.w1 * sv);
             if (u2) buf_atomic_add(rs, o2, s.w2 * sv);
@@ -305,6 +317,7 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
             acc += (float)((int)u1 + (int)u2 + (int)u3);
         }
         a0 = a1; b0 = b1; fla0 = fla1; flb0 = flb1;
+        tb += w.st;
     }
     return acc;
 }
@@ -344,13 +357,11 @@ __device__ float project_all_dominances(const Launch& P, const Ray& r, bool insi
     return acc;
 }
 
+// One workgroup of 64 x 4 rays (columns ia, rows ib) of view iv.
 template <int MODE, bool REF>
-__global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __restrict__ pv_in,
-                                                     float* __restrict__ pv_out, int32_t* __restrict__ counts)
+__device__ __forceinline__ void siddon_rays(const Launch& P, int ia, int ib, int iv, const float* __restrict__ pv_in,
+                                            float* __restrict__ pv_out, int32_t* __restrict__ counts)
 {
-    const int ia = blockIdx.x * BX + threadIdx.x;
-    const int ib = blockIdx.y * BY + threadIdx.y;
-    const int iv = blockIdx.z;
     const bool inside = ia < P.G.na && ib < P.G.nb;
     const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
     Ray r;
@@ -366,6 +377,14 @@ __global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __res
     if (!inside) return;
     if (MODE == 0) pv_out[id] = acc * r.scale;
     if (MODE == 2) counts[id] = (int32_t)acc;
+}
+
+template <int MODE, bool REF>
+__global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __restrict__ pv_in,
+                                                     float* __restrict__ pv_out, int32_t* __restrict__ counts)
+{
+    siddon_rays<MODE, REF>(P, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z, pv_in,
+                           pv_out, counts);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,6 +603,62 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
 }
 
 // ---------------------------------------------------------------------------
+// Per-ray parameter table: the FP64 set-up of every ray of a chunk of views,
+// packed into 32 bytes, so that consumers that visit a ray many times (the
+// volume-tile adjoint) read the forward's exact FP32 parameters instead of
+// recomputing them.  tc is (s0 + s1) >> 1 and the slope signs come from
+// ka, kb, so 8 words suffice.
+// ---------------------------------------------------------------------------
+struct RayRec {
+    float4 p0;  // Ac, ka, Bc, kb
+    float4 p1;  // inv_ka, inv_kb, scale, bits(s0 | s1 << 15 | dom << 30)
+};
+
+__device__ __forceinline__ RayRec pack_ray(const Ray& r)
+{
+    RayRec q;
+    q.p0 = make_float4(r.Ac, r.ka, r.Bc, r.kb);
+    const unsigned dom = (r.dom < 0) ? 3u : (unsigned)r.dom;
+    const unsigned bits = (unsigned)r.s0 | ((unsigned)r.s1 << 15) | (dom << 30);
+    q.p1 = make_float4(r.inv_ka, r.inv_kb, r.scale, __uint_as_float(bits));
+    return q;
+}
+
+__device__ __forceinline__ void unpack_ray(const RayRec& q, Ray& r)
+{
+    r.Ac = q.p0.x;
+    r.ka = q.p0.y;
+    r.Bc = q.p0.z;
+    r.kb = q.p0.w;
+    r.inv_ka = q.p1.x;
+    r.inv_kb = q.p1.y;
+    r.scale = q.p1.z;
+    const unsigned bits = __float_as_uint(q.p1.w);
+    r.s0 = (int)(bits & 0x7fffu);
+    r.s1 = (int)((bits >> 15) & 0x7fffu);
+    const unsigned dom = bits >> 30;
+    r.dom = (dom == 3u) ? -1 : (int)dom;
+    r.tc = (r.s0 + r.s1) >> 1;
+    // ka and 1/ka share their sign; one of them is a normal float whenever
+    // the slope is non-zero, so the test below matches the FP64 one.
+    r.pa = (r.ka > 0.0f || r.inv_ka > 0.0f) ? 1.0f : 0.0f;
+    r.pb = (r.kb > 0.0f || r.inv_kb > 0.0f) ? 1.0f : 0.0f;
+}
+
+// table[(iv - v0) * nb * na + ib * na + ia] for views [v0, v0 + nvc)
+__global__ __launch_bounds__(256) void k_ray_table(Launch P, int v0, RayRec* __restrict__ table, int adjoint)
+{
+    const int ia = blockIdx.x * 256 + threadIdx.x;
+    const int ib = blockIdx.y;
+    const int iv = v0 + blockIdx.z;
+    if (ia >= P.G.na) return;
+    const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
+    Ray r;
+    setup_ray(P.V, P.G, c, s, ia, ib, P.flags, adjoint != 0, r);
+    table[((size_t)blockIdx.z * P.G.nb + ib) * P.G.na + ia] = pack_ray(r);
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint owned by volume tiles (no global atomics).
 //
 // A workgroup owns a TX x TY x TZ block of voxels held in LDS.  For every view
@@ -644,30 +719,41 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
 }
 
 template <bool REF, int TX, int TY, int TZ, int NW>
-__global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
-                                                             float* __restrict__ image)
+__device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
+                                               int bx, int by, int bz, TileAcc<TX, TY, TZ>& acc,
+                                               const RayRec* __restrict__ table = nullptr, int v0 = 0, int v1 = -1,
+                                               bool accumulate = false)
 {
-    __shared__ TileAcc<TX, TY, TZ> acc;
     constexpr int SX = TileAcc<TX, TY, TZ>::SX;
     const int tid = threadIdx.x;
     float* af = &acc.v[0][0][0];
+    __syncthreads();   // the previous block's write-out has read acc
     for (int i = tid; i < TZ * TY * SX; i += 64 * NW) af[i] = 0.0f;
     const int nx = P.V.n[0], ny = P.V.n[1], nz = P.V.n[2];
-    const int b0[3] = {(int)blockIdx.x * TX, (int)blockIdx.y * TY, (int)blockIdx.z * TZ};
+    const int b0[3] = {bx * TX, by * TY, bz * TZ};
     const int ext[3] = {min(TX, nx - b0[0]), min(TY, ny - b0[1]), min(TZ, nz - b0[2])};
     const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
     __syncthreads();
-    for (int iv = 0; iv < P.nv; ++iv) {
+    if (v1 < 0) v1 = P.nv;
+    for (int iv = v0; iv < v1; ++iv) {
         const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
         int ia0, ia1, ib0, ib1;
         shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
         if (ia1 < ia0 || ib1 < ib0) continue;
-        const int wa = ia1 - ia0 + 1;
-        const int ncand = wa * (ib1 - ib0 + 1);
-        for (int k = tid; k < ncand; k += 64 * NW) {
-            const int ib = ib0 + k / wa, ia = ia0 + k % wa;
+        // wavefront w takes detector rows ib0 + w, ib0 + w + NW, ...; its lanes
+        // take neighbouring columns, so a wavefront's rays are nearly parallel
+        const int wv = tid >> 6, ln = tid & 63;
+        for (int k = 0; k < (ia1 - ia0 + 64) / 64 * ((ib1 - ib0 + NW) / NW); ++k) {
+            const int cols = (ia1 - ia0 + 64) / 64;
+            const int ib = ib0 + wv + NW * (k / cols), ia = ia0 + ln + 64 * (k % cols);
+            if (ib > ib1 || ia > ia1) continue;
             Ray r;
-            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
+            if (table) {
+                const RayRec q = table[((size_t)(iv - v0) * P.G.nb + ib) * P.G.na + ia];
+                unpack_ray(q, r);
+            } else {
+                setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
+            }
             if (r.dom < 0) continue;
             const int m = r.dom, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
             // slabs of the block on the walk axis, within the ray's own range
@@ -734,10 +820,21 @@ __global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const flo
     __syncthreads();
     for (int i = tid; i < TX * TY * TZ; i += 64 * NW) {
         const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
-        if (lx < ext[0] && ly < ext[1] && lz < ext[2])
-            image[(size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz))] =
-                acc.v[lz][ly][lx];
+        if (lx < ext[0] && ly < ext[1] && lz < ext[2]) {
+            const size_t o = (size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz));
+            image[o] = accumulate ? image[o] + acc.v[lz][ly][lx] : acc.v[lz][ly][lx];
+        }
     }
+}
+
+template <bool REF, int TX, int TY, int TZ, int NW>
+__global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
+                                                             float* __restrict__ image, const RayRec* __restrict__ table,
+                                                             int v0, int v1, int accumulate)
+{
+    __shared__ TileAcc<TX, TY, TZ> acc;
+    adj_tile_block<REF, TX, TY, TZ, NW>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, table, v0, v1,
+                                        accumulate != 0);
 }
 
 __global__ void k_view_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
@@ -823,7 +920,7 @@ __global__ __launch_bounds__(256) void k_combine(const float* __restrict__ xf, c
     }
 }
 
-inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+__host__ __device__ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct WsPlan {
     size_t trig_off, xf_off, yf_off, total;
@@ -831,7 +928,7 @@ struct WsPlan {
     int NXP, NYP, NZP;
 };
 
-WsPlan plan_ws(const int grid[3], int nv)
+__host__ __device__ WsPlan plan_ws(const int grid[3], int nv)
 {
     WsPlan p;
     const size_t nx = grid[0], ny = grid[1], nz = grid[2];
@@ -869,7 +966,7 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
     if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
     if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT | NEFCT_F_DETERMINISTIC |
-                  NEFCT_F_ADJ_WINDOW))
+                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_RECOMPUTE))
         return NEFCT_ERR_UNSUPPORTED;
     if ((flags & NEFCT_F_DETERMINISTIC) && (flags & NEFCT_F_ADJ_WINDOW)) return NEFCT_ERR_UNSUPPORTED;
     if (flags & NEFCT_F_REF_SLAB) {
@@ -888,6 +985,7 @@ Launch make_launch(const int grid[3], const float center[3], const float size[3]
     for (int i = 0; i < 3; ++i) {
         L.V.n[i] = grid[i];
         L.V.vs[i] = (double)size[i] / grid[i];
+        L.V.inv_vs[i] = 1.0 / L.V.vs[i];
         L.V.lo[i] = (double)center[i] - (double)size[i] * 0.5;
     }
     L.G.cyl = cyl;
@@ -985,22 +1083,181 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
         hipLaunchKernelGGL(k_combine, dim3((nx + 63) / 64, (ny + 63) / 64, nz), dim3(256), 0, s,
                            (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP,
                            image);
-    } else if (flags & NEFCT_F_DETERMINISTIC) {
-        // one wavefront per 16x16x8 block: fixed LDS summation order
-        const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + 7) / 8);
-        if (ref)
-            hipLaunchKernelGGL((k_siddon_adj_tile<true, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image);
-        else
-            hipLaunchKernelGGL((k_siddon_adj_tile<false, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image);
     } else {
-        const dim3 g((nx + 31) / 32, (ny + 31) / 32, (nz + 7) / 8);
-        if (ref)
-            hipLaunchKernelGGL((k_siddon_adj_tile<true, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image);
-        else
-            hipLaunchKernelGGL((k_siddon_adj_tile<false, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image);
+        // Volume blocks own their voxels.  Unless told to recompute, the
+        // per-ray set-up is written once per chunk of views into the
+        // workspace (which the padded copies of the forward occupy) and the
+        // blocks read it; a chunk's blocks add to the previous chunks' sums.
+        const bool det = (flags & NEFCT_F_DETERMINISTIC) != 0;
+        const size_t per_view = sizeof(RayRec) * (size_t)na * (size_t)nb;
+        const size_t cap = p.total - p.xf_off;
+        int chunk = (int)std::min<size_t>((size_t)nv, cap / per_view);
+        if (flags & NEFCT_F_ADJ_RECOMPUTE) chunk = 0;
+        RayRec* table = (RayRec*)(ws + p.xf_off);
+        auto tile = [&](const RayRec* tb, int v0, int v1, int acc) {
+            if (det) {
+                // one wavefront per 16x16x8 block: fixed LDS summation order
+                const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + 7) / 8);
+                if (ref)
+                    hipLaunchKernelGGL((k_siddon_adj_tile<true, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image, tb,
+                                       v0, v1, acc);
+                else
+                    hipLaunchKernelGGL((k_siddon_adj_tile<false, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image, tb,
+                                       v0, v1, acc);
+            } else {
+                const dim3 g((nx + 31) / 32, (ny + 31) / 32, (nz + 7) / 8);
+                if (ref)
+                    hipLaunchKernelGGL((k_siddon_adj_tile<true, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image, tb,
+                                       v0, v1, acc);
+                else
+                    hipLaunchKernelGGL((k_siddon_adj_tile<false, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image,
+                                       tb, v0, v1, acc);
+            }
+        };
+        if (chunk <= 0) {
+            tile(nullptr, 0, nv, 0);
+        } else {
+            for (int v0 = 0; v0 < nv; v0 += chunk) {
+                const int v1 = std::min(nv, v0 + chunk);
+                hipLaunchKernelGGL(k_ray_table, dim3((na + 255) / 256, nb, v1 - v0), dim3(256), 0, s, L, v0, table,
+                                   1);
+                tile(table, v0, v1, v0 > 0 ? 1 : 0);
+            }
+        }
     }
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched problems (persistent kernels fed by an atomic task queue)
+// ---------------------------------------------------------------------------
+constexpr int BT = 32, BTZ = 8;          // adjoint block (same as the plain default)
+constexpr int Q_HEADER = 256;            // workspace head: task counters
+
+__device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
+{
+    Launch L;
+    for (int i = 0; i < 3; ++i) {
+        L.V.n[i] = pr.grid[i];
+        L.V.vs[i] = (double)pr.size[i] / pr.grid[i];
+        L.V.inv_vs[i] = 1.0 / L.V.vs[i];
+        L.V.lo[i] = (double)pr.center[i] - (double)pr.size[i] * 0.5;
+    }
+    L.G.cyl = pr.cyl;
+    L.G.sid = pr.SID;
+    L.G.sad = pr.SAD;
+    L.G.na = pr.na;
+    L.G.nb = pr.nb;
+    L.G.da = pr.da;
+    L.G.ai = pr.ai;
+    L.G.db = pr.db;
+    L.G.bi = pr.bi;
+    const WsPlan p = plan_ws(pr.grid, pr.nv);
+    char* ws = ws_base + pr.ws_offset;
+    L.Ly.xf = (float*)(ws + p.xf_off);
+    L.Ly.yf = (float*)(ws + p.yf_off);
+    L.Ly.xf_bytes = (uint32_t)(p.xf_elems * 4);
+    L.Ly.yf_bytes = (uint32_t)(p.yf_elems * 4);
+    L.Ly.NXP = p.NXP;
+    L.Ly.NYP = p.NYP;
+    L.Ly.NZP = p.NZP;
+    L.trig = (const double*)(ws + p.trig_off);
+    L.nv = pr.nv;
+    L.flags = flags;
+    return L;
+}
+
+__device__ __forceinline__ int find_problem(const nefct_problem* __restrict__ probs, int nprob, long long t,
+                                            bool blocks)
+{
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const long long b = blocks ? probs[mid].block_begin : probs[mid].tile_begin;
+        if (b <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// trig tables of every problem + zeroed task counters
+__global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob, char* ws_base)
+{
+    const int p = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && i < 8) ((unsigned long long*)ws_base)[i] = 0ull;
+    if (p >= nprob) return;
+    const nefct_problem& pr = probs[p];
+    if (i >= pr.nv) return;
+    const WsPlan pl = plan_ws(pr.grid, pr.nv);
+    double* trig = (double*)(ws_base + pr.ws_offset + pl.trig_off);
+    const double a = (double)pr.angles[i];
+    trig[2 * i] = cos(a);
+    trig[2 * i + 1] = sin(a);
+}
+
+template <bool REF>
+__global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __restrict__ probs, int nprob,
+                                                       long long total, char* ws_base, unsigned flags)
+{
+    __shared__ long long s_task;
+    unsigned long long* counter = (unsigned long long*)ws_base;
+    for (;;) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) s_task = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long t = s_task;
+        __syncthreads();
+        if (t >= total) break;
+        const int p = find_problem(probs, nprob, t, false);
+        const nefct_problem& pr = probs[p];
+        const long long local = t - pr.tile_begin;
+        const long long per_view = (long long)pr.tiles_a * pr.tiles_b;
+        const int iv = (int)(local / per_view);
+        const int rem = (int)(local - (long long)iv * per_view);
+        const int tb = rem / pr.tiles_a, ta = rem - tb * pr.tiles_a;
+        const Launch L = launch_of(pr, ws_base, flags);
+        siddon_rays<0, REF>(L, ta * BX + threadIdx.x, tb * BY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
+    }
+}
+
+template <bool REF>
+__global__ __launch_bounds__(256) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
+                                                   long long total, char* ws_base, unsigned flags)
+{
+    __shared__ TileAcc<BT, BT, BTZ> acc;
+    __shared__ long long s_task;
+    unsigned long long* counter = (unsigned long long*)ws_base + 1;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long t = s_task;
+        __syncthreads();
+        if (t >= total) break;
+        const int p = find_problem(probs, nprob, t, true);
+        const nefct_problem& pr = probs[p];
+        const int local = (int)(t - pr.block_begin);
+        const int bx = local % pr.blocks[0];
+        const int by = (local / pr.blocks[0]) % pr.blocks[1];
+        const int bz = local / (pr.blocks[0] * pr.blocks[1]);
+        const Launch L = launch_of(pr, ws_base, flags);
+        adj_tile_block<REF, BT, BT, BTZ, 4>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc);
+    }
+}
+
+// Workgroups of a persistent launch: per_cu per compute unit of the current
+// device (queried once per device; safe to call while a graph is captured).
+int persistent_grid(int per_cu)
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * per_cu;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev] = cus;
+    }
+    return cached[dev] * per_cu;
 }
 
 }  // namespace
@@ -1081,6 +1338,90 @@ int nefct_siddon_count(const int grid[3], const float center[3], const float siz
         hipLaunchKernelGGL((k_siddon<2, true>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
     else
         hipLaunchKernelGGL((k_siddon<2, false>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* workspace_bytes, long long* total_tiles,
+                     long long* total_blocks)
+{
+    if (!probs || !workspace_bytes || !total_tiles || !total_blocks) return NEFCT_ERR_NULL;
+    if (nprob <= 0 || nprob > 65535) return NEFCT_ERR_SHAPE;
+    long long tiles = 0, blocks = 0;
+    size_t off = Q_HEADER;
+    for (int i = 0; i < nprob; ++i) {
+        nefct_problem& pr = probs[i];
+        int st = validate(pr.grid, pr.center, pr.size, pr.angles, pr.nv, pr.SID, pr.SAD, pr.na, pr.da, pr.nb, pr.db,
+                          pr.ai, pr.bi, flags);
+        if (st) return st;
+        if (!pr.volume || !pr.sinogram) return NEFCT_ERR_NULL;
+        pr.tiles_a = (pr.na + BX - 1) / BX;
+        pr.tiles_b = (pr.nb + BY - 1) / BY;
+        pr.tile_begin = tiles;
+        tiles += (long long)pr.tiles_a * pr.tiles_b * pr.nv;
+        pr.blocks[0] = (pr.grid[0] + BT - 1) / BT;
+        pr.blocks[1] = (pr.grid[1] + BT - 1) / BT;
+        pr.blocks[2] = (pr.grid[2] + BTZ - 1) / BTZ;
+        pr.block_begin = blocks;
+        blocks += (long long)pr.blocks[0] * pr.blocks[1] * pr.blocks[2];
+        pr.ws_offset = (long long)off;
+        off += align256(plan_ws(pr.grid, pr.nv).total);
+        pr.pad_ = 0;
+    }
+    *workspace_bytes = off;
+    *total_tiles = tiles;
+    *total_blocks = blocks;
+    return NEFCT_OK;
+}
+
+static int batch_prep(const nefct_problem* host, const nefct_problem* dev, int nprob, char* ws, hipStream_t s)
+{
+    int maxnv = 1;
+    for (int i = 0; i < nprob; ++i) maxnv = max(maxnv, host[i].nv);
+    hipLaunchKernelGGL(k_batch_prep, dim3((maxnv + 255) / 256, nprob), dim3(256), 0, s, dev, nprob, ws);
+    return NEFCT_OK;
+}
+
+int nefct_batch_project(const nefct_problem* host, const nefct_problem* dev, int nprob, long long total_tiles,
+                        unsigned flags, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!host || !dev || !workspace) return NEFCT_ERR_NULL;
+    if (nprob <= 0) return NEFCT_ERR_SHAPE;
+    const nefct_problem& last = host[nprob - 1];
+    if (workspace_bytes < (size_t)last.ws_offset + align256(plan_ws(last.grid, last.nv).total))
+        return NEFCT_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    batch_prep(host, dev, nprob, ws, s);
+    for (int i = 0; i < nprob; ++i) {
+        const WsPlan p = plan_ws(host[i].grid, host[i].nv);
+        launch_pad(host[i].volume, host[i].grid, p, ws + host[i].ws_offset, s);
+    }
+    const int g = persistent_grid(8);
+    if (flags & NEFCT_F_REF_SLAB)
+        hipLaunchKernelGGL((k_batch_fwd<true>), dim3(g), dim3(BX, BY), 0, s, dev, nprob, total_tiles, ws, flags);
+    else
+        hipLaunchKernelGGL((k_batch_fwd<false>), dim3(g), dim3(BX, BY), 0, s, dev, nprob, total_tiles, ws, flags);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev, int nprob, long long total_blocks,
+                            unsigned flags, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!host || !dev || !workspace) return NEFCT_ERR_NULL;
+    if (nprob <= 0) return NEFCT_ERR_SHAPE;
+    const nefct_problem& last = host[nprob - 1];
+    if (workspace_bytes < (size_t)last.ws_offset + align256(plan_ws(last.grid, last.nv).total))
+        return NEFCT_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    batch_prep(host, dev, nprob, ws, s);
+    const int g = persistent_grid(4);
+    if (flags & NEFCT_F_REF_SLAB)
+        hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
+    else
+        hipLaunchKernelGGL((k_batch_adj<false>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }

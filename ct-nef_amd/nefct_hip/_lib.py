@@ -19,6 +19,7 @@ F_SHORT_SKIP = 4
 F_TIE_GT = 8
 F_DETERMINISTIC = 16
 F_ADJ_WINDOW = 32
+F_ADJ_RECOMPUTE = 64
 REF_FORWARD = F_INV_L2 | F_REF_SLAB
 REF_ADJOINT = F_REF_SLAB | F_SHORT_SKIP | F_TIE_GT
 
@@ -46,6 +47,9 @@ SIGNATURES = {
     "nefct_fdk_filter": [_vp, _vp, _i, _i, _vp, _vp],
     "nefct_fdk_workspace_size": [_i],
     "nefct_fdk_backproject": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp, _sz, _vp],
+    "nefct_batch_plan": [_vp, _i, _u, _vp, _vp, _vp],
+    "nefct_batch_project": [_vp, _vp, _i, ctypes.c_longlong, _u, _vp, _sz, _vp],
+    "nefct_batch_backproject": [_vp, _vp, _i, ctypes.c_longlong, _u, _vp, _sz, _vp],
 }
 _RESTYPE = {
     "nefct_status_string": ctypes.c_char_p,

@@ -58,3 +58,47 @@ def short_scan_views(wl: Workload):
     step = 2 * math.pi / wl.nv
     n = int(math.ceil(span / step)) + 1
     return np.arange(min(n, wl.nv))
+
+
+@dataclass(frozen=True)
+class PhaseProblem:
+    """One problem of a batched workload: geometry, its own angles, and the
+    seeds of its phantom and displacement field."""
+    vol: VolumeGeometry
+    det: ConeGeometry
+    angles: np.ndarray
+    phantom_seed: int
+    dvf_seed: int
+
+
+def c4(seed: int = 4):
+    """Ten respiratory phases of one 256x256x160 @ 1.0 mm base volume; phase
+    views uniform in [40, 120], angles uniform over 360 deg; 384 x 288 detector
+    (pitch 1.0 x 0.833 mm covers the volume at SDD/SOD = 1.5); each phase has
+    its own Gaussian-bump displacement field (dvf_seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vol = VolumeGeometry.cube((256, 256, 160), 1.0)
+    det = ConeGeometry.flat(1000.0, 1500.0, 384, 1.5 * 256 / 384, 288, 1.5 * 160 / 288)
+    out = []
+    for p in range(10):
+        nv = int(rng.integers(40, 121))
+        ang = np.sort(rng.uniform(0, 2 * math.pi, nv)).astype(np.float32)
+        out.append(PhaseProblem(vol, det, ang, seed, 1000 * seed + p))
+    return out
+
+
+def c5(seed: int = 5, count: int = 64):
+    """64 independent problems: cubic side in {64, 96, 128, 192, 256} @ 1 mm,
+    views uniform in [60, 360] over 360 deg, detector (2 side)^2 at pitch 0.75 mm,
+    SOD uniform in [500, 1000], SDD = 1.5 SOD; own phantom and warp field each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for p in range(count):
+        side = int(rng.choice([64, 96, 128, 192, 256]))
+        nv = int(rng.integers(60, 361))
+        sod = float(rng.uniform(500.0, 1000.0))
+        vol = VolumeGeometry.cube((side, side, side), 1.0)
+        det = ConeGeometry.flat(sod, 1.5 * sod, 2 * side, 0.75, 2 * side, 0.75)
+        ang = (2 * math.pi * np.arange(nv) / nv).astype(np.float32)
+        out.append(PhaseProblem(vol, det, ang, 100 * seed + p, 200 * seed + p))
+    return out

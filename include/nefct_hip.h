@@ -52,11 +52,16 @@ typedef enum {
 #define NEFCT_F_SHORT_SKIP 4u /* Q3: adjoint skips rays with in-plane length < 10 voxels (:472) */
 #define NEFCT_F_TIE_GT 8u     /* Q6: adjoint x-walk only if |xd| > |yd| (:480) */
 /* Adjoint algorithm selection (ignored by the forward):
- *   default                 volume blocks own their voxels in LDS (no global atomics)
+ *   default                 volume blocks own their voxels in LDS (no global atomics);
+ *                           per-ray parameters come from a table that a set-up
+ *                           kernel writes into the workspace, a chunk of views at a time
  *   NEFCT_F_DETERMINISTIC   one wavefront per block: bitwise reproducible result
- *   NEFCT_F_ADJ_WINDOW      ray tiles with LDS slab windows flushed by global atomics */
+ *   NEFCT_F_ADJ_WINDOW      ray tiles with LDS slab windows flushed by global atomics
+ *   NEFCT_F_ADJ_RECOMPUTE   volume blocks recompute each ray's set-up (no table)
+ * All variants give the same weights; only the summation order differs. */
 #define NEFCT_F_DETERMINISTIC 16u
 #define NEFCT_F_ADJ_WINDOW 32u
+#define NEFCT_F_ADJ_RECOMPUTE 64u
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
 
@@ -110,6 +115,53 @@ int nefct_siddon_count(const int grid[3], const float center[3], const float siz
                        const float* angles, int nv, float SID, float SAD, int na, float da, float ai,
                        int nb, float db, float bi, int cyl, int32_t* counts, unsigned flags,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Batched Siddon pair: many problems of uneven size in one persistent     */
+/* launch per direction.  No reference counterpart: it replaces the        */
+/* reference's per-view Python loops (nefct/functions/project_t.py:153-159, */
+/* back_project_t.py:150-156) and per-problem launches.                     */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    /* set by the caller (device pointers, siddon-op geometry) */
+    const float* volume;  /* forward input / adjoint output, x fastest */
+    float* sinogram;      /* forward output / adjoint input, [nv][nb][na] */
+    const float* angles;  /* [nv] */
+    int grid[3];
+    float center[3];
+    float size[3];
+    int nv;
+    int cyl;
+    float SID, SAD;
+    int na, nb;
+    float da, ai, db, bi;
+    /* filled by nefct_batch_plan */
+    long long tile_begin;   /* first forward task (64x4 rays of one view) */
+    long long block_begin;  /* first adjoint task (one 32x32x8 voxel block) */
+    long long ws_offset;    /* byte offset of the problem's workspace region */
+    int tiles_a, tiles_b;
+    int blocks[3];
+    int pad_;
+} nefct_problem;
+
+/* Validates the problems, fills their plan fields and returns the workspace
+ * size and task totals.  Host-only; no device work. */
+int nefct_batch_plan(nefct_problem* problems, int nprob, unsigned flags, size_t* workspace_bytes,
+                     long long* total_tiles, long long* total_blocks);
+
+/* Forward projection of every problem: one trig/counter pass, the padded
+ * volume copies, then one persistent kernel whose workgroups take 64x4-ray
+ * tasks from an atomic queue in the workspace.  'host_problems' is the
+ * planned table, 'device_problems' a device copy of it. */
+int nefct_batch_project(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
+                        long long total_tiles, unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Exact (or reference-mode) adjoint of every problem: one persistent kernel
+ * whose workgroups take 32x32x8 voxel blocks from an atomic queue.  Every
+ * output volume is fully written. */
+int nefct_batch_backproject(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
+                            long long total_blocks, unsigned flags, void* workspace, size_t workspace_bytes,
+                            void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Trilinear warp                                                          */

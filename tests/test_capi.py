@@ -57,3 +57,22 @@ def test_operators_refuse_cpu_tensors():
     det = nh.ConeGeometry.flat(100.0, 150.0, 4, 1.0, 4, 1.0)
     with pytest.raises(nh.NefctError):
         nh.project(torch.zeros(vol.shape_zyx), vol, det, [0.0])
+
+
+def test_problem_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of nefct_problem has the C layout (compiled with gcc)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    from nefct_hip.batch import Problem
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "nefct_hip.h"\n'
+                   'int main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(nefct_problem),'
+                   ' offsetof(nefct_problem, tile_begin), offsetof(nefct_problem, blocks),'
+                   ' offsetof(nefct_problem, SID), offsetof(nefct_problem, ws_offset)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [ctypes.sizeof(Problem), Problem.tile_begin.offset, Problem.blocks.offset, Problem.SID.offset,
+                   Problem.ws_offset.offset]

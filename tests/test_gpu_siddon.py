@@ -227,11 +227,16 @@ def test_autograd_matches_adjoint():
     assert torch.allclose(x.grad, ref, rtol=1e-5, atol=1e-6 * ref.abs().max().item())
 
 
-@pytest.mark.parametrize("algo", ["tile", "deterministic", "window"])
+@pytest.mark.parametrize("algo", ["tile", "recompute", "deterministic", "deterministic-recompute", "window"])
 def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
+    """Every adjoint variant against the FP64 transpose.  At these sizes the
+    per-ray table of the tile variants holds 4-6 views, so the table path runs
+    over several chunks of views that add to one another."""
     nh = _nh()
     from nefct_hip import _lib
-    fl = {"tile": 0, "deterministic": _lib.F_DETERMINISTIC, "window": _lib.F_ADJ_WINDOW}[algo]
+    fl = {"tile": 0, "recompute": _lib.F_ADJ_RECOMPUTE, "deterministic": _lib.F_DETERMINISTIC,
+          "deterministic-recompute": _lib.F_DETERMINISTIC | _lib.F_ADJ_RECOMPUTE,
+          "window": _lib.F_ADJ_WINDOW}[algo]
     for gi in (0, 1, 2):
         name, vol, det = _geoms()[gi]
         _, ang, img, ov, od = _case(oracle, vol, det, 16, seed=gi + 21)
