@@ -59,6 +59,78 @@ def cpu_baseline(wl, img_cpu, stride):
     }
 
 
+def batched_leg(dev, iters, log_every=True):
+    """64 uneven differentiable problems (sides 64-256, 60-360 views, detector
+    2 side, SOD 500-1000 mm, own phantom and Gaussian-bump warp field each):
+    one gradient pass = warp + forward + adjoint with gradients to the base
+    volume and the displacement field, and the projector pair alone.  The
+    plain path calls the per-problem autograd functions in a loop; the batched
+    path runs one persistent launch per direction over all problems."""
+    import torch
+    import nefct_hip as nh
+    from nefct_hip.batch import SiddonBatch, batched_project
+    from nefct_hip.configs import c5
+    from nefct_hip.deform import warp
+    from nefct_hip.fourd import warp_project
+    from nefct_hip.phantom import ellipsoid_phantom, gaussian_dvf
+
+    probs = c5()
+    log(f"[bench] batched leg: {len(probs)} problems, building phantoms and warp fields")
+    bases = [ellipsoid_phantom(p.vol, p.phantom_seed, 10, device=dev) for p in probs]
+    dvfs = [gaussian_dvf(p.vol.shape_zyx, p.dvf_seed, device=dev) for p in probs]
+    angs = [torch.tensor(p.angles, device=dev) for p in probs]
+    batch = SiddonBatch([(p.vol, p.det, p.angles) for p in probs], device=dev)
+    g = torch.Generator(device=dev).manual_seed(55)
+    ys = [torch.randn(s.shape, generator=g, device=dev) for s in batch.sinograms]
+    rays = batch.rays
+
+    def plain(with_warp):
+        for p, b, d, a, y in zip(probs, bases, dvfs, angs, ys):
+            x = b.detach().requires_grad_(True)
+            if with_warp:
+                dv = d.detach().requires_grad_(True)
+                v = warp(x, dv, 1)
+            else:
+                v = x
+            s = nh.forward_project(v, p.vol, p.det, a)
+            (s * y).sum().backward()
+
+    def batched(with_warp):
+        xs = [b.detach().requires_grad_(True) for b in bases]
+        if with_warp:
+            ds = [d.detach().requires_grad_(True) for d in dvfs]
+            outs = warp_project(batch, xs, ds, boundary=1)
+        else:
+            outs = batched_project(batch, xs)
+        sum((o * y).sum() for o, y in zip(outs, ys)).backward()
+
+    res = {}
+    for name, fn, ww in (("plain_pair", plain, False), ("batched_pair", batched, False),
+                         ("plain_warp_pair", plain, True), ("batched_warp_pair", batched, True)):
+        fn(ww)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn(ww)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / iters
+        res[name] = rays / dt / 1e9
+        log(f"[bench] batched leg {name}: {dt * 1e3:.1f} ms per pass, {res[name]:.4f} Grays/s")
+    return {
+        "workload": "64 uneven problems: cubic sides {64..256} @ 1 mm, 60-360 views, detector (2 side)^2, "
+                    "SOD 500-1000 mm; gradient pass to volume (and warp field)",
+        "rays": rays,
+        "unit": "Grays/s",
+        "plain_pair": round(res["plain_pair"], 5),
+        "batched_pair": round(res["batched_pair"], 5),
+        "speedup_pair": round(res["batched_pair"] / res["plain_pair"], 3),
+        "plain_warp_pair": round(res["plain_warp_pair"], 5),
+        "batched_warp_pair": round(res["batched_warp_pair"], 5),
+        "speedup_warp_pair": round(res["batched_warp_pair"] / res["plain_warp_pair"], 3),
+        "iters": iters,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +144,9 @@ def main():
                     help="profiling only: adjoint algorithm flags (16 deterministic, 32 slab windows)")
     ap.add_argument("--only", choices=["pair", "fwd", "adj"], default="pair",
                     help="profiling only: run one direction")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain leg")
+    ap.add_argument("--batched-iters", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -87,6 +162,9 @@ def main():
     wl = c2()
     vol, det = wl.vol, wl.det
     dev = torch.device("cuda:0")
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(dev, args.batched_iters)}), flush=True)
+        return
     log(f"[bench] workload {wl.name}: building phantom")
     img = ellipsoid_phantom(vol, wl.seed, wl.ellipsoids, device=dev)
     ang = torch.tensor(wl.angles(), device=dev)
@@ -139,6 +217,12 @@ def main():
     b_dom = bytes_adj if t_adj > t_fwd else bytes_fwd
     achieved = b_dom / t_dom / 1e9
 
+    batched = None
+    if not args.no_batched and not args.views and args.only == "pair":
+        del sino_fixed
+        torch.cuda.empty_cache()
+        batched = batched_leg(dev, args.batched_iters)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline on views 0::{args.cpu_stride}")
@@ -180,6 +264,7 @@ def main():
             "algorithmic_bytes": b_dom,
             "avg_launch_ms": round(t_dom * 1e3, 3),
         },
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -681,44 +681,49 @@ struct TileAcc {
 __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
                                               const int hi[3], int& ia0, int& ia1, int& ib0, int& ib1)
 {
-    // fractional detector indices of the 8 corners (flat: a = t SDD / U, b = z SDD / U;
-    // cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)); margin of 1 cell.
-    float fa_mn = 1e30f, fa_mx = -1e30f, fb_mn = 1e30f, fb_mx = -1e30f;
-    bool behind = false;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const double x = V.lo[0] + V.vs[0] * ((q & 1) ? hi[0] : lo[0]);
-        const double y = V.lo[1] + V.vs[1] * ((q & 2) ? hi[1] : lo[1]);
-        const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]);
-        const double U = G.sid + x * c + y * s;
-        const double t = -x * s + y * c;
-        if (U <= 1e-6 * G.sid) behind = true;
-        double a, b;
-        if (G.cyl) {
-            a = atan2(t, U);
-            b = z * G.sad / sqrt(U * U + t * t);
-        } else {
-            a = t * G.sad / U;
-            b = z * G.sad / U;
-        }
-        const float fa = (float)((a + G.na * G.da * 0.5) / G.da - G.ai);
-        const float fb = (float)((b + G.nb * G.db * 0.5) / G.db - G.bi);
-        fa_mn = fminf(fa_mn, fa);
-        fa_mx = fmaxf(fa_mx, fa);
-        fb_mn = fminf(fb_mn, fb);
-        fb_mx = fmaxf(fb_mx, fb);
+    // Fractional detector indices of the 8 corners (flat: a = t SDD / U,
+    // b = z SDD / U; cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)),
+    // with a margin of 1 cell.  Lane l computes corner l & 7 and the groups of
+    // 8 lanes reduce by butterfly, so every lane ends with the same bounds;
+    // readfirstlane makes them scalar.
+    const int q = threadIdx.x & 7;
+    const double x = V.lo[0] + V.vs[0] * ((q & 1) ? hi[0] : lo[0]);
+    const double y = V.lo[1] + V.vs[1] * ((q & 2) ? hi[1] : lo[1]);
+    const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]);
+    const double U = G.sid + x * c + y * s;
+    const double t = -x * s + y * c;
+    int behind = (U <= 1e-6 * G.sid) ? 1 : 0;
+    double a, b;
+    if (G.cyl) {
+        a = atan2(t, U);
+        b = z * G.sad / sqrt(U * U + t * t);
+    } else {
+        const double iu = G.sad / U;
+        a = t * iu;
+        b = z * iu;
     }
-    if (behind) {  // block reaches behind the source: take the whole detector
+    float fa_mn = (float)((a + G.na * G.da * 0.5) / G.da - G.ai);
+    float fb_mn = (float)((b + G.nb * G.db * 0.5) / G.db - G.bi);
+    float fa_mx = fa_mn, fb_mx = fb_mn;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        fa_mn = fminf(fa_mn, __shfl_xor(fa_mn, m));
+        fa_mx = fmaxf(fa_mx, __shfl_xor(fa_mx, m));
+        fb_mn = fminf(fb_mn, __shfl_xor(fb_mn, m));
+        fb_mx = fmaxf(fb_mx, __shfl_xor(fb_mx, m));
+        behind |= __shfl_xor(behind, m);
+    }
+    if (__builtin_amdgcn_readfirstlane(behind)) {  // block reaches behind the source: take the whole detector
         ia0 = 0; ia1 = G.na - 1; ib0 = 0; ib1 = G.nb - 1;
         return;
     }
-    ia0 = max(0, (int)floorf(fa_mn) - 1);
-    ia1 = min(G.na - 1, (int)ceilf(fa_mx) + 1);
-    ib0 = max(0, (int)floorf(fb_mn) - 1);
-    ib1 = min(G.nb - 1, (int)ceilf(fb_mx) + 1);
+    ia0 = __builtin_amdgcn_readfirstlane(max(0, (int)floorf(fa_mn) - 1));
+    ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, (int)ceilf(fa_mx) + 1));
+    ib0 = __builtin_amdgcn_readfirstlane(max(0, (int)floorf(fb_mn) - 1));
+    ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, (int)ceilf(fb_mx) + 1));
 }
 
-template <bool REF, int TX, int TY, int TZ, int NW>
+template <bool REF, int TX, int TY, int TZ, int NW, int CS = 1, bool TABLE = false>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
                                                int bx, int by, int bz, TileAcc<TX, TY, TZ>& acc,
                                                const RayRec* __restrict__ table = nullptr, int v0 = 0, int v1 = -1,
@@ -741,14 +746,20 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
         if (ia1 < ia0 || ib1 < ib0) continue;
         // wavefront w takes detector rows ib0 + w, ib0 + w + NW, ...; its lanes
-        // take neighbouring columns, so a wavefront's rays are nearly parallel
+        // take columns CS apart (CS passes cover a 64 CS column chunk), so a
+        // wavefront's rays are nearly parallel; with CS = 2 neighbouring lanes
+        // are about one voxel apart rather than half a voxel, and their LDS
+        // additions rarely meet on one address
         const int wv = tid >> 6, ln = tid & 63;
-        for (int k = 0; k < (ia1 - ia0 + 64) / 64 * ((ib1 - ib0 + NW) / NW); ++k) {
-            const int cols = (ia1 - ia0 + 64) / 64;
-            const int ib = ib0 + wv + NW * (k / cols), ia = ia0 + ln + 64 * (k % cols);
+        const int cols = CS * ((ia1 - ia0 + 64 * CS) / (64 * CS));
+        const int nk = cols * ((ib1 - ib0 + NW) / NW);
+        for (int k = 0; k < nk; ++k) {
+            const int kc = k % cols;
+            const int ib = ib0 + wv + NW * (k / cols);
+            const int ia = ia0 + (kc / CS) * (64 * CS) + CS * ln + (kc % CS);
             if (ib > ib1 || ia > ia1) continue;
             Ray r;
-            if (table) {
+            if (TABLE) {
                 const RayRec q = table[((size_t)(iv - v0) * P.G.nb + ib) * P.G.na + ia];
                 unpack_ray(q, r);
             } else {
@@ -827,14 +838,35 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     }
 }
 
-template <bool REF, int TX, int TY, int TZ, int NW>
+template <bool REF, int TX, int TY, int TZ, int NW, int CS, bool TABLE>
 __global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
                                                              float* __restrict__ image, const RayRec* __restrict__ table,
                                                              int v0, int v1, int accumulate)
 {
     __shared__ TileAcc<TX, TY, TZ> acc;
-    adj_tile_block<REF, TX, TY, TZ, NW>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, table, v0, v1,
-                                        accumulate != 0);
+    adj_tile_block<REF, TX, TY, TZ, NW, CS, TABLE>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, table, v0,
+                                                   v1, accumulate != 0);
+}
+
+// Launch of one tile-adjoint variant over views [v0, v1).
+template <int TX, int TY, int TZ, int NW, int CS>
+void launch_tile(bool ref, const Launch& L, const float* pv, float* image, const RayRec* table, int v0, int v1,
+                 int acc, hipStream_t s)
+{
+    const dim3 g((L.V.n[0] + TX - 1) / TX, (L.V.n[1] + TY - 1) / TY, (L.V.n[2] + TZ - 1) / TZ);
+    const dim3 b(64 * NW);
+    if (ref && table)
+        hipLaunchKernelGGL((k_siddon_adj_tile<true, TX, TY, TZ, NW, CS, true>), g, b, 0, s, L, pv, image, table, v0,
+                           v1, acc);
+    else if (ref)
+        hipLaunchKernelGGL((k_siddon_adj_tile<true, TX, TY, TZ, NW, CS, false>), g, b, 0, s, L, pv, image, table, v0,
+                           v1, acc);
+    else if (table)
+        hipLaunchKernelGGL((k_siddon_adj_tile<false, TX, TY, TZ, NW, CS, true>), g, b, 0, s, L, pv, image, table, v0,
+                           v1, acc);
+    else
+        hipLaunchKernelGGL((k_siddon_adj_tile<false, TX, TY, TZ, NW, CS, false>), g, b, 0, s, L, pv, image, table, v0,
+                           v1, acc);
 }
 
 __global__ void k_view_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
@@ -966,7 +998,7 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
     if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
     if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT | NEFCT_F_DETERMINISTIC |
-                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_RECOMPUTE))
+                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_RECOMPUTE | NEFCT_F_TILE_MASK | NEFCT_F_LANE_STRIDE2))
         return NEFCT_ERR_UNSUPPORTED;
     if ((flags & NEFCT_F_DETERMINISTIC) && (flags & NEFCT_F_ADJ_WINDOW)) return NEFCT_ERR_UNSUPPORTED;
     if (flags & NEFCT_F_REF_SLAB) {
@@ -1094,24 +1126,24 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
         int chunk = (int)std::min<size_t>((size_t)nv, cap / per_view);
         if (flags & NEFCT_F_ADJ_RECOMPUTE) chunk = 0;
         RayRec* table = (RayRec*)(ws + p.xf_off);
+        const unsigned shape = flags & NEFCT_F_TILE_MASK;
+        const bool cs2 = (flags & NEFCT_F_LANE_STRIDE2) != 0;
         auto tile = [&](const RayRec* tb, int v0, int v1, int acc) {
             if (det) {
                 // one wavefront per 16x16x8 block: fixed LDS summation order
-                const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + 7) / 8);
-                if (ref)
-                    hipLaunchKernelGGL((k_siddon_adj_tile<true, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image, tb,
-                                       v0, v1, acc);
-                else
-                    hipLaunchKernelGGL((k_siddon_adj_tile<false, 16, 16, 8, 1>), g, dim3(64), 0, s, L, pv, image, tb,
-                                       v0, v1, acc);
+                launch_tile<16, 16, 8, 1, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
+            } else if (shape == NEFCT_F_TILE_32x32x12) {
+                if (cs2) launch_tile<32, 32, 12, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<32, 32, 12, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
+            } else if (shape == NEFCT_F_TILE_64x16x8) {
+                if (cs2) launch_tile<64, 16, 8, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<64, 16, 8, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
+            } else if (shape == NEFCT_F_TILE_16x16x32) {
+                if (cs2) launch_tile<16, 16, 32, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<16, 16, 32, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
             } else {
-                const dim3 g((nx + 31) / 32, (ny + 31) / 32, (nz + 7) / 8);
-                if (ref)
-                    hipLaunchKernelGGL((k_siddon_adj_tile<true, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image, tb,
-                                       v0, v1, acc);
-                else
-                    hipLaunchKernelGGL((k_siddon_adj_tile<false, 32, 32, 8, 4>), g, dim3(256), 0, s, L, pv, image,
-                                       tb, v0, v1, acc);
+                if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<32, 32, 8, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
             }
         };
         if (chunk <= 0) {
@@ -1133,7 +1165,7 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
 // Batched problems (persistent kernels fed by an atomic task queue)
 // ---------------------------------------------------------------------------
 constexpr int BT = 32, BTZ = 8;          // adjoint block (same as the plain default)
-constexpr int Q_HEADER = 256;            // workspace head: task counters
+constexpr int Q_HEADER = 256;            // workspace head: task counters, then one Launch per problem
 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
 {
@@ -1180,14 +1212,23 @@ __device__ __forceinline__ int find_problem(const nefct_problem* __restrict__ pr
     return lo;
 }
 
-// trig tables of every problem + zeroed task counters
-__global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob, char* ws_base)
+__host__ __device__ inline size_t batch_launch_bytes(int nprob) { return align256(sizeof(Launch) * (size_t)nprob); }
+
+// Per-problem launch parameters live in the workspace: a kernel that picks its
+// problem at run time reads them from global memory (uniform scalar loads)
+// instead of building a private copy, which would be indexed dynamically and
+// so be placed in scratch memory.
+__device__ __forceinline__ const Launch* batch_launches(char* ws_base) { return (const Launch*)(ws_base + Q_HEADER); }
+
+// trig tables and launch parameters of every problem + zeroed task counters
+__global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob, char* ws_base, unsigned flags)
 {
     const int p = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0 && i < 8) ((unsigned long long*)ws_base)[i] = 0ull;
     if (p >= nprob) return;
     const nefct_problem& pr = probs[p];
+    if (i == 0) ((Launch*)(ws_base + Q_HEADER))[p] = launch_of(pr, ws_base, flags);
     if (i >= pr.nv) return;
     const WsPlan pl = plan_ws(pr.grid, pr.nv);
     double* trig = (double*)(ws_base + pr.ws_offset + pl.trig_off);
@@ -1208,14 +1249,14 @@ __global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __re
         const long long t = s_task;
         __syncthreads();
         if (t >= total) break;
-        const int p = find_problem(probs, nprob, t, false);
+        const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, false));
         const nefct_problem& pr = probs[p];
         const long long local = t - pr.tile_begin;
         const long long per_view = (long long)pr.tiles_a * pr.tiles_b;
         const int iv = (int)(local / per_view);
         const int rem = (int)(local - (long long)iv * per_view);
         const int tb = rem / pr.tiles_a, ta = rem - tb * pr.tiles_a;
-        const Launch L = launch_of(pr, ws_base, flags);
+        const Launch& L = batch_launches(ws_base)[p];
         siddon_rays<0, REF>(L, ta * BX + threadIdx.x, tb * BY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
     }
 }
@@ -1233,13 +1274,13 @@ __global__ __launch_bounds__(256) void k_batch_adj(const nefct_problem* __restri
         const long long t = s_task;
         __syncthreads();
         if (t >= total) break;
-        const int p = find_problem(probs, nprob, t, true);
+        const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, true));
         const nefct_problem& pr = probs[p];
         const int local = (int)(t - pr.block_begin);
         const int bx = local % pr.blocks[0];
         const int by = (local / pr.blocks[0]) % pr.blocks[1];
         const int bz = local / (pr.blocks[0] * pr.blocks[1]);
-        const Launch L = launch_of(pr, ws_base, flags);
+        const Launch& L = batch_launches(ws_base)[p];
         adj_tile_block<REF, BT, BT, BTZ, 4>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc);
     }
 }
@@ -1348,7 +1389,7 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
     if (!probs || !workspace_bytes || !total_tiles || !total_blocks) return NEFCT_ERR_NULL;
     if (nprob <= 0 || nprob > 65535) return NEFCT_ERR_SHAPE;
     long long tiles = 0, blocks = 0;
-    size_t off = Q_HEADER;
+    size_t off = Q_HEADER + batch_launch_bytes(nprob);
     for (int i = 0; i < nprob; ++i) {
         nefct_problem& pr = probs[i];
         int st = validate(pr.grid, pr.center, pr.size, pr.angles, pr.nv, pr.SID, pr.SAD, pr.na, pr.da, pr.nb, pr.db,
@@ -1374,11 +1415,12 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
     return NEFCT_OK;
 }
 
-static int batch_prep(const nefct_problem* host, const nefct_problem* dev, int nprob, char* ws, hipStream_t s)
+static int batch_prep(const nefct_problem* host, const nefct_problem* dev, int nprob, char* ws, unsigned flags,
+                      hipStream_t s)
 {
     int maxnv = 1;
     for (int i = 0; i < nprob; ++i) maxnv = max(maxnv, host[i].nv);
-    hipLaunchKernelGGL(k_batch_prep, dim3((maxnv + 255) / 256, nprob), dim3(256), 0, s, dev, nprob, ws);
+    hipLaunchKernelGGL(k_batch_prep, dim3((maxnv + 255) / 256, nprob), dim3(256), 0, s, dev, nprob, ws, flags);
     return NEFCT_OK;
 }
 
@@ -1392,7 +1434,7 @@ int nefct_batch_project(const nefct_problem* host, const nefct_problem* dev, int
         return NEFCT_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
-    batch_prep(host, dev, nprob, ws, s);
+    batch_prep(host, dev, nprob, ws, flags, s);
     for (int i = 0; i < nprob; ++i) {
         const WsPlan p = plan_ws(host[i].grid, host[i].nv);
         launch_pad(host[i].volume, host[i].grid, p, ws + host[i].ws_offset, s);
@@ -1416,7 +1458,7 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
         return NEFCT_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
-    batch_prep(host, dev, nprob, ws, s);
+    batch_prep(host, dev, nprob, ws, flags, s);
     const int g = persistent_grid(4);
     if (flags & NEFCT_F_REF_SLAB)
         hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);

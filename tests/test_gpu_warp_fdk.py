@@ -75,7 +75,7 @@ def test_warp_dot_test_and_autograd(boundary):
     (Warp.apply(xi, di, boundary) * y).sum().backward()
     assert torch.allclose(xi.grad, WTy, atol=1e-6, rtol=1e-5)
     # displacement gradient by central differences inside one cell
-    frac = torch.frac(dvf)
+    frac = dvf - torch.floor(dvf)                # in [0, 1), also for negative displacements
     base = torch.floor(dvf) + 0.2 + 0.6 * frac   # keep fractional parts in [0.2, 0.8]
     dv = base.clone().requires_grad_(True)
     (Warp.apply(x, dv, boundary) * y).sum().backward()
