@@ -143,3 +143,38 @@ def fdk(sino: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, park
     q = preweight(sino, det, parker=parker, dang_rad=dang, direction=direction, scale=scale)
     q = ramp_filter(q, window)
     return backproject(q, vol, det, angles, offsets)
+
+
+def footprint_backproject(filtered: torch.Tensor, shape, angles, offsets=None, *, SID: float, SAD: float,
+                          da: float, ai: float, na: int, db: float, bi: float, nb: int, cyl: bool = False,
+                          out: torch.Tensor = None) -> torch.Tensor:
+    """Pixel-driven FDK backprojection with slab footprints, the arithmetic of
+    nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:17-255 with that file's units
+    and naming: lengths in voxel units, volume centred on the axis,
+    SAD = source-to-axis and SID = source-to-detector distance.
+    ``filtered`` [nv, nb, na]; ``shape`` = (nx, ny, nz); ``offsets`` [nv] or
+    None.  Adds into ``out`` ([nz, ny, nx], zero-filled when not given), as
+    the reference launcher does."""
+    _need_device(filtered)
+    L = _lib.load()
+    filtered = filtered.contiguous()
+    ang = torch.as_tensor(angles, dtype=torch.float32, device=filtered.device).contiguous()
+    nv = ang.numel()
+    if filtered.numel() != nv * nb * na:
+        raise _lib.NefctError(f"projections have {filtered.numel()} elements, expected {nv}x{nb}x{na}")
+    nx, ny, nz = (int(v) for v in shape)
+    if out is None:
+        out = torch.zeros((nz, ny, nx), dtype=torch.float32, device=filtered.device)
+    elif out.shape != (nz, ny, nx) or not out.is_contiguous() or out.dtype != torch.float32:
+        raise _lib.NefctError("out must be a contiguous float32 [nz, ny, nx] tensor")
+    off = None
+    if offsets is not None:
+        off = torch.as_tensor(offsets, dtype=torch.float32, device=filtered.device).contiguous()
+        if off.numel() != nv:
+            raise _lib.NefctError("one offset per view")
+    fn = L.nefct_fdk_footprint_backproject_cyli if cyl else L.nefct_fdk_footprint_backproject_flat
+    st = fn(_ptr(filtered), _lib.int3((nx, ny, nz)), None if off is None else _ptr(off), _ptr(ang), nv,
+            float(SID), float(SAD), float(da), float(ai), int(na), float(db), float(bi), int(nb), _ptr(out),
+            ctypes.c_void_p(_stream_ptr(filtered.device)))
+    _lib.check(st, "fdk_footprint_backproject")
+    return out

@@ -230,6 +230,40 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
                           float ai, int nb, float db, float bi, float* image, void* workspace,
                           size_t workspace_bytes, void* stream);
 
+/* Voxel-driven backprojection without weights: the reference's second
+ * BackProjectFlat / BackProjectCyli op pair (nefct/cc/pixel_3d/pixel.cc:11-47,
+ * launchers pixel.cu.cc:126-204, kernels :12-124).  image[v] = sum over views
+ * of the bilinear sample of projection_value at the central projection of
+ * voxel centre v; views whose 2 x 2 neighbourhood leaves the detector are
+ * skipped.  The reference's conventions define the op: voxel centre
+ * (i + 0.5) size/n + center (no -size/2), source at distance SID from the
+ * axis, detector at SAD from the source, index y/d + n/2 (integer n/2)
+ * - offset - 0.5.  Host grid/center/size; device projection [nv][nb][na],
+ * angles [nv] (radians), image [nz][ny][nx] (fully written). */
+int nefct_pixel_backproject_flat(const float* projection_value, const int grid[3], const float center[3],
+                                 const float size[3], const float* angles, int nv, float SID, float SAD, int na,
+                                 float da, float ai, int nb, float db, float bi, float* image, void* stream);
+int nefct_pixel_backproject_cyli(const float* projection_value, const int grid[3], const float center[3],
+                                 const float size[3], const float* angles, int nv, float SID, float SAD, int na,
+                                 float da, float ai, int nb, float db, float bi, float* image, void* stream);
+
+/* Pixel-driven FDK backprojection with slab footprints: the arithmetic of
+ * nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:17-255; replaces its launchers
+ * back_project_flat_fdk_gpu (:257-276) and back_project_cyli_fdk_gpu
+ * (:278-298), same argument order.  That file's units and naming: lengths in
+ * voxel units, volume centred on the axis, SAD = source-to-axis distance,
+ * SID = source-to-detector distance, da in voxel units (flat) or radians
+ * (cylindrical).  shape = (nx, ny, nz) on the host (the reference copies it
+ * from the device); offsets[nv] (may be NULL = 0) lift source and detector
+ * along z.  Adds into image [nz][ny][nx], as the reference does (the caller
+ * zeroes it first); additions are atomic, so the summation order varies. */
+int nefct_fdk_footprint_backproject_flat(const float* pv_values, const int shape[3], const float* offsets,
+                                         const float* angles, int nv, float SID, float SAD, float da, float ai,
+                                         int na, float db, float bi, int nb, float* image, void* stream);
+int nefct_fdk_footprint_backproject_cyli(const float* pv_values, const int shape[3], const float* offsets,
+                                         const float* angles, int nv, float SID, float SAD, float da, float ai,
+                                         int na, float db, float bi, int nb, float* image, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

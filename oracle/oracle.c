@@ -287,3 +287,153 @@ int orc_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------
+ * Voxel-driven backprojection without weights, restated in FP64 from
+ * nefct/cc/pixel_3d/pixel.cu.cc:12-124 (flat :12-52, cylindrical :54-95,
+ * voxel centres :116-118).  The reference's integer n/2 in the detector
+ * index is kept.  image [nz][ny][nx] is overwritten.
+ * ------------------------------------------------------------------------ */
+void orc_pixel_bp(int cyl, const int grid[3], const double center[3], const double size[3],
+                  const double *angles, int nv, double SID, double SAD, int na, double da, double ai,
+                  int nb, double db, double bi, const double *pv, double *image)
+{
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    const double ha = (double)(na / 2), hb = (double)(nb / 2);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const double xc = (ix + 0.5) * size[0] / nx + center[0];
+                const double yc = (iy + 0.5) * size[1] / ny + center[1];
+                const double zc = (iz + 0.5) * size[2] / nz + center[2];
+                double acc = 0.0;
+                for (int iv = 0; iv < nv; ++iv) {
+                    const double c = cos(angles[iv]), s = sin(angles[iv]);
+                    const double x0 = xc * c + yc * s;
+                    const double y0 = -xc * s + yc * c;
+                    const double mag = SAD / (x0 + SID);
+                    const double a = cyl ? atan2(y0, x0 + SID) / da + ha - ai - 0.5
+                                         : mag * y0 / da + ha - ai - 0.5;
+                    const double b = mag * zc / db + hb - bi - 0.5;
+                    const int ia = (int)floor(a), ib = (int)floor(b);
+                    if (ia < 0 || ia + 1 >= na || ib < 0 || ib + 1 >= nb) continue;
+                    const double wa2 = a - ia, wa = 1.0 - wa2, wb2 = b - ib, wb = 1.0 - wb2;
+                    const double *p = pv + (size_t)iv * na * nb;
+                    acc += p[ia + (size_t)ib * na] * wa * wb + p[ia + (size_t)(ib + 1) * na] * wa * wb2 +
+                           p[ia + 1 + (size_t)ib * na] * wa2 * wb + p[ia + 1 + (size_t)(ib + 1) * na] * wa2 * wb2;
+                }
+                image[ix + (size_t)nx * (iy + (size_t)ny * iz)] = acc;
+            }
+}
+
+/* ------------------------------------------------------------------------
+ * Pixel-driven FDK backprojection with slab footprints, restated in FP64
+ * from nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:17-255.  Lengths are in
+ * voxel units with the volume centred on the axis (the reference's Python
+ * layer divides by the voxel size); that file names the source-to-axis
+ * distance SAD and the source-to-detector distance SID.  For every pixel
+ * (ia, ib, iv) and every slab of its dominant in-plane axis, the pixel's
+ * footprint on the slab's mid-plane is bounded by its two edge rays in the
+ * transverse in-plane axis (all four slope combinations, :104-125) and by
+ * the centre ray +- db/2 in z (:127-138); each overlapped voxel receives
+ * pv * wy * wz * |r| / |r_dom| * SAD^2 / (SAD + x cos + y sin)^2 (:142-157).
+ * offsets[iv] (NULL = 0) raises source and detector along z.  The integer
+ * divisions n/2 of the reference are kept where it has them.  image
+ * [nz][ny][nx] is overwritten.
+ * ------------------------------------------------------------------------ */
+static void fdk_fp_edge(int cyl, double SID, double SAD, double da, double ai, int na, double e, double c,
+                        double s, double x1, double y1, double *xo, double *yo)
+{
+    double x20, y20;
+    if (!cyl) {
+        x20 = SID - SAD;
+        y20 = e * da + ai - na * da / 2;
+    } else {
+        const double ang = e * da + ai - na * da / 2;
+        x20 = SID * cos(ang) - SAD;
+        y20 = SID * sin(ang);
+    }
+    *xo = x20 * c - y20 * s - x1;
+    *yo = x20 * s + y20 * c - y1;
+}
+
+void orc_fdk_fp_bp(int cyl, int nx, int ny, int nz, const double *angles, const double *offsets, int nv,
+                   double SID, double SAD, int na, double da, double ai, int nb, double db, double bi,
+                   const double *pv, double *image)
+{
+    const size_t nvox = (size_t)nx * ny * nz;
+    memset(image, 0, sizeof(double) * nvox);
+    const int hx = nx / 2, hy = ny / 2, hz = nz / 2;
+#pragma omp parallel
+    {
+        double *priv = (double *)calloc(nvox, sizeof(double));
+#pragma omp for collapse(2) schedule(dynamic, 1)
+        for (int iv = 0; iv < nv; ++iv)
+            for (int ib = 0; ib < nb; ++ib)
+                for (int ia = 0; ia < na; ++ia) {
+                    const double c = cos(angles[iv]), s = sin(angles[iv]);
+                    const double off = offsets ? offsets[iv] : 0.0;
+                    const double x1 = -SAD * c, y1 = -SAD * s, z1 = off;
+                    double x20, y20;
+                    if (!cyl) {
+                        x20 = SID - SAD;
+                        y20 = (ia + 0.5) * da + ai - na * da / 2;
+                    } else {
+                        const double ang = (ia + 0.5) * da + ai - na * da / 2;
+                        x20 = SID * cos(ang) - SAD;
+                        y20 = SID * sin(ang);
+                    }
+                    const double z20 = (ib + 0.5) * db + bi - nb * db / 2;
+                    const double x21 = x20 * c - y20 * s - x1;
+                    const double y21 = x20 * s + y20 * c - y1;
+                    const double z21 = z20 + off - z1;
+                    const double val = pv[ia + (size_t)na * (ib + (size_t)nb * iv)];
+                    const double len = sqrt(x21 * x21 + y21 * y21 + z21 * z21);
+                    double xl, yl, xr, yr;
+                    fdk_fp_edge(cyl, SID, SAD, da, ai, na, (double)ia, c, s, x1, y1, &xl, &yl);
+                    fdk_fp_edge(cyl, SID, SAD, da, ai, na, (double)(ia + 1), c, s, x1, y1, &xr, &yr);
+                    const int xdom = fabs(x21) > fabs(y21);
+                    const int nm = xdom ? nx : ny;      /* walk axis */
+                    for (int im = 0; im < nm; ++im) {
+                        /* distance of the slab mid-plane from the source along the walk axis */
+                        const double mc = xdom ? (im + 0.5 - nx / 2.0 - x1) : (im + 0.5 - ny / 2.0 - y1);
+                        double q[4];
+                        if (xdom) {
+                            q[0] = yl / xl * mc; q[1] = yl / xr * mc; q[2] = yr / xl * mc; q[3] = yr / xr * mc;
+                        } else {
+                            q[0] = xl / yl * mc; q[1] = xl / yr * mc; q[2] = xr / yl * mc; q[3] = xr / yr * mc;
+                        }
+                        const double base_t = xdom ? (y1 + hy) : (x1 + hx);
+                        double t1 = q[0] + base_t, t2 = t1;
+                        for (int k = 1; k < 4; ++k) {
+                            const double v = q[k] + base_t;
+                            if (v < t1) t1 = v;
+                            if (v > t2) t2 = v;
+                        }
+                        const double rdom = xdom ? x21 : y21;
+                        const double zl = (z21 - db / 2) / rdom * mc + z1 + hz;
+                        const double zr = (z21 + db / 2) / rdom * mc + z1 + hz;
+                        const double z1f = zl < zr ? zl : zr, z2f = zl < zr ? zr : zl;
+                        const int T1 = (int)floor(t1), T2 = (int)floor(t2);
+                        const int Z1 = (int)floor(z1f), Z2 = (int)floor(z2f);
+                        const int nt = xdom ? ny : nx;
+                        for (int it = (T1 > 0 ? T1 : 0); it <= (T2 < nt - 1 ? T2 : nt - 1); ++it) {
+                            const double wt = ((it + 1.0 < t2 ? it + 1.0 : t2) - (it > t1 ? it : t1)) / (t2 - t1);
+                            for (int iz = (Z1 > 0 ? Z1 : 0); iz <= (Z2 < nz - 1 ? Z2 : nz - 1); ++iz) {
+                                const double wz = ((iz + 1.0 < z2f ? iz + 1.0 : z2f) - (iz > z1f ? iz : z1f)) /
+                                                  (z2f - z1f);
+                                const int ix = xdom ? im : it, iy = xdom ? it : im;
+                                const double xc = ix + 0.5 - hx, yc = iy + 0.5 - hy;
+                                const double u = SAD + xc * c + yc * s;
+                                const double w = wt * wz / fabs(rdom) * len * SAD * SAD / u / u;
+                                priv[ix + (size_t)nx * (iy + (size_t)ny * iz)] += val * w;
+                            }
+                        }
+                    }
+                }
+#pragma omp critical
+        for (size_t i = 0; i < nvox; ++i) image[i] += priv[i];
+        free(priv);
+    }
+}

@@ -93,6 +93,8 @@ def lib():
         L.orc_ref_fwd_f64.argtypes = [P(_Vol), P(_Det), P(d), i, P(d), P(d), ctypes.c_uint, i, i]
         L.orc_ref_bp_f64.argtypes = [P(_Vol), P(_Det), P(d), i, P(d), P(d), ctypes.c_uint, i, i]
         L.orc_threads.restype = i
+        L.orc_pixel_bp.argtypes = [i, P(i), P(d), P(d), P(d), i, d, d, i, d, d, i, d, d, P(d), P(d)]
+        L.orc_fdk_fp_bp.argtypes = [i, i, i, i, P(d), P(d), i, d, d, i, d, d, i, d, d, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -453,4 +455,43 @@ def tv_adjoint(d):
         sl_lo[ax] = slice(None, -1)
         out[tuple(sl_lo)] -= d[ax][tuple(sl_hi)]
         out[tuple(sl_hi)] += d[ax][tuple(sl_hi)]
+    return out
+
+
+# --------------------------------------------------------------------------
+# Voxel-driven unweighted backprojection (nefct/cc/pixel_3d/pixel.cu.cc:12-124)
+# and pixel-driven FDK footprint backprojection
+# (nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:17-255), C restatements in FP64
+# --------------------------------------------------------------------------
+def pixel_bp(cyl, grid, center_low, size, angles, SID, SAD, na, da, ai, nb, db, bi, pv):
+    """grid (nx, ny, nz); center_low = the reference op's ``center`` (voxel
+    centres at (i + 0.5) size / n + center); pv [nv, nb, na] -> [nz, ny, nx]."""
+    nx, ny, nz = (int(v) for v in grid)
+    ang = np.ascontiguousarray(angles, np.float64)
+    pv = np.ascontiguousarray(pv, np.float64)
+    assert pv.shape == (ang.size, nb, na)
+    out = np.empty((nz, ny, nx), np.float64)
+    g = np.array([nx, ny, nz], np.int32)
+    c = np.ascontiguousarray(center_low, np.float64)
+    z = np.ascontiguousarray(size, np.float64)
+    lib().orc_pixel_bp(int(bool(cyl)), _ptr(g, ctypes.c_int), _ptr(c, ctypes.c_double), _ptr(z, ctypes.c_double),
+                       _ptr(ang, ctypes.c_double), ang.size, float(SID), float(SAD), int(na), float(da), float(ai),
+                       int(nb), float(db), float(bi), _ptr(pv, ctypes.c_double), _ptr(out, ctypes.c_double))
+    return out
+
+
+def fdk_fp_bp(cyl, grid, angles, offsets, SID, SAD, na, da, ai, nb, db, bi, pv):
+    """Footprint FDK backprojection in voxel units, naming as in
+    dist_3d_back_fdk.cu.cc (SAD = source-axis, SID = source-detector);
+    pv [nv, nb, na] -> [nz, ny, nx]."""
+    nx, ny, nz = (int(v) for v in grid)
+    ang = np.ascontiguousarray(angles, np.float64)
+    pv = np.ascontiguousarray(pv, np.float64)
+    assert pv.shape == (ang.size, nb, na)
+    off = None if offsets is None else np.ascontiguousarray(offsets, np.float64)
+    out = np.empty((nz, ny, nx), np.float64)
+    lib().orc_fdk_fp_bp(int(bool(cyl)), nx, ny, nz, _ptr(ang, ctypes.c_double),
+                        None if off is None else _ptr(off, ctypes.c_double), ang.size, float(SID), float(SAD),
+                        int(na), float(da), float(ai), int(nb), float(db), float(bi), _ptr(pv, ctypes.c_double),
+                        _ptr(out, ctypes.c_double))
     return out
