@@ -1,0 +1,153 @@
+"""Autotuner for the Siddon adjoint's launch shape.
+
+The exact adjoint has several implementations with identical weights that
+differ in how work maps onto the chip (include/nefct_hip.h, adjoint flags):
+volume blocks of 32x32x8, 32x32x12, 64x16x8 or 16x16x32 voxels, neighbouring
+lanes one or two detector columns apart, per-ray parameters read from a
+table or recomputed, or ray tiles with LDS slab windows.  Which is fastest
+depends on the volume, detector and view count, so the tuner times the
+candidates on a sample of the views (timing scales with the view count) and
+keeps the winner per (device, geometry, mode) in a JSON cache.
+
+Cache file: ``$NEFCT_AUTOTUNE_CACHE`` or ``~/.cache/nefct_hip/autotune.json``.
+Entries are read-mostly; writes replace the file atomically.
+"""
+from __future__ import annotations
+
+import json
+import os
+import tempfile
+import threading
+from typing import Dict, Optional, Tuple
+
+import torch
+
+from . import _lib
+from .geometry import ConeGeometry, VolumeGeometry
+
+ADJOINT_CANDIDATES: Dict[str, int] = {
+    "tile32x32x8": _lib.F_TILE_32x32x8,
+    "tile32x32x8/stride2": _lib.F_TILE_32x32x8 | _lib.F_LANE_STRIDE2,
+    "tile32x32x8/recompute": _lib.F_TILE_32x32x8 | _lib.F_ADJ_RECOMPUTE,
+    "tile32x32x12": _lib.F_TILE_32x32x12,
+    "tile32x32x12/stride2": _lib.F_TILE_32x32x12 | _lib.F_LANE_STRIDE2,
+    "tile64x16x8": _lib.F_TILE_64x16x8,
+    "tile64x16x8/stride2": _lib.F_TILE_64x16x8 | _lib.F_LANE_STRIDE2,
+    "tile16x16x32": _lib.F_TILE_16x16x32,
+    "window": _lib.F_ADJ_WINDOW,
+}
+
+_lock = threading.Lock()
+_cache: Optional[dict] = None
+
+
+def cache_path() -> str:
+    p = os.environ.get("NEFCT_AUTOTUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "nefct_hip", "autotune.json")
+
+
+def _load() -> dict:
+    global _cache
+    if _cache is None:
+        try:
+            with open(cache_path()) as f:
+                _cache = json.load(f)
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def _store(key: str, value: dict):
+    with _lock:
+        c = _load()
+        c[key] = value
+        path = cache_path()
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            fd, tmp = tempfile.mkstemp(dir=os.path.dirname(path), prefix=".autotune")
+            with os.fdopen(fd, "w") as f:
+                json.dump(c, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        except OSError:
+            pass   # an unwritable cache only costs re-tuning
+
+
+def _views_bucket(nv: int) -> int:
+    b = 1
+    while b < nv:
+        b *= 2
+    return b
+
+
+def key_of(vol: VolumeGeometry, det: ConeGeometry, nv: int, mode_flags: int, device) -> str:
+    props = torch.cuda.get_device_properties(device)
+    arch = getattr(props, "gcnArchName", props.name)
+    vs = ",".join(f"{v:.4g}" for v in vol.voxel)
+    return (f"{arch}|adjoint|grid={vol.grid}|vox={vs}|det={det.mode},{det.na}x{det.nb},"
+            f"SID={det.SID:.4g},SAD={det.SAD:.4g},da={det.da:.4g}|views~{_views_bucket(nv)}|mode={mode_flags}")
+
+
+def _time_ms(fn, reps: int) -> float:
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_adjoint(vol: VolumeGeometry, det: ConeGeometry, angles, mode_flags: int = 0,
+                 candidates: Optional[Dict[str, int]] = None, sample_views: int = 64, reps: int = 2,
+                 force: bool = False) -> Tuple[int, dict]:
+    """Pick the fastest adjoint variant for this geometry.  ``mode_flags``
+    are the semantic flags (0 exact, or REF_ADJOINT) kept in every
+    candidate.  Returns (flags, record); the record lists every candidate's
+    time on the view sample."""
+    from .siddon import backproject
+    ang = torch.as_tensor(angles, dtype=torch.float32).reshape(-1)
+    nv = ang.numel()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    key = key_of(vol, det, nv, mode_flags, dev)
+    if not force:
+        rec = _load().get(key)
+        if rec is not None:
+            return int(rec["flags"]), rec
+    step = max(1, nv // sample_views)
+    sub = ang[::step][:sample_views].to(dev)
+    g = torch.Generator(device=dev).manual_seed(0)
+    y = torch.rand((sub.numel(), det.nb, det.na), generator=g, device=dev)
+    cands = dict(candidates or ADJOINT_CANDIDATES)
+    times = {}
+    for name, fl in cands.items():
+        flags = int(mode_flags) | int(fl)
+        times[name] = _time_ms(lambda: backproject(y, vol, det, sub, flags=flags), reps)
+    best = min(times, key=times.get)
+    rec = {"flags": int(mode_flags) | int(cands[best]), "variant": best, "sample_views": int(sub.numel()),
+           "ms_on_sample": {k: round(v, 3) for k, v in times.items()}}
+    _store(key, rec)
+    return int(rec["flags"]), rec
+
+
+def tuned_flags(vol: VolumeGeometry, det: ConeGeometry, nv: int, mode_flags: int = 0) -> int:
+    """Flags of the cached winner for this geometry, or the mode flags alone
+    (default variant) when it has not been tuned."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rec = _load().get(key_of(vol, det, nv, mode_flags, dev))
+    return int(rec["flags"]) if rec else int(mode_flags)
+
+
+def backproject_tuned(sino: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, mode_flags: int = 0,
+                      tune: bool = True) -> torch.Tensor:
+    """``backproject`` with the tuned variant (tuning on first use)."""
+    from .siddon import backproject
+    if tune:
+        flags, _ = tune_adjoint(vol, det, angles, mode_flags)
+    else:
+        flags = tuned_flags(vol, det, torch.as_tensor(angles).numel(), mode_flags)
+    return backproject(sino, vol, det, angles, flags=flags)

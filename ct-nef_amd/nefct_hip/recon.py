@@ -106,6 +106,40 @@ def sart(b, vol, det, angles, iters: int = 10, relax: float = 1.0, x0=None):
     return x
 
 
+def conj_grad(b, vol, det, angles, n_iter: int = 10, mu: float = 0.0, d=None, x0=None):
+    """Conjugate gradients on the normal equations of
+    min 0.5 ||A x - b||^2 + 0.5 mu ||W x - d||^2 (W = tv_diff), i.e.
+    (A^T A + mu W^T W) x = A^T b + mu W^T d, as nefct/functions/conj_grad.py:37-62
+    (``d`` = None drops the TV term's data, mu = 0 the TV term).  Inner
+    products in FP64."""
+    dev = b.device
+    x = torch.zeros(vol.shape_zyx, dtype=torch.float32, device=dev) if x0 is None else x0.clone()
+
+    def normal(v):
+        out = backproject(project(v, vol, det, angles), vol, det, angles)
+        if mu:
+            out = out + mu * tv_diff_adjoint(tv_diff(v))
+        return out
+
+    rhs = backproject(b, vol, det, angles)
+    if mu and d is not None:
+        rhs = rhs + mu * tv_diff_adjoint(d)
+    r = rhs - normal(x)
+    p = r.clone()
+    rs_old = float((r.double() * r.double()).sum())
+    for _ in range(n_iter):
+        if rs_old == 0.0:
+            break
+        Ap = normal(p)
+        alpha = rs_old / float((p.double() * Ap.double()).sum())
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rs_new = float((r.double() * r.double()).sum())
+        p = r + (rs_new / rs_old) * p
+        rs_old = rs_new
+    return x
+
+
 def finite_difference_check(x, b, vol, det, angles, voxels, eps_fd: float = 1e-3, lam: float = 1e-3,
                             eps: float = 1e-6):
     """Central differences of ``loss`` at the given flat voxel indices against

@@ -50,3 +50,31 @@ def test_gradient_descent_and_sart_reduce_the_residual():
     r0 = torch.linalg.vector_norm(b).item()
     r1 = torch.linalg.vector_norm(nh.project(xs, vol, det, ang) - b).item()
     assert r1 < 0.3 * r0
+
+
+def test_conj_grad_approaches_the_dense_normal_equation_solution():
+    """Tiny problem: the normal matrix A^T A + mu W^T W is assembled column by
+    column from the GPU operators (FP64 on the host) and solved densely; CG
+    (conj_grad.py:37-62) reaches that solution."""
+    import nefct_hip as nh
+    from nefct_hip.recon import conj_grad, tv_diff, tv_diff_adjoint
+    vol = nh.VolumeGeometry.cube((6, 6, 5), 2.0)
+    det = nh.ConeGeometry.flat(100.0, 150.0, 16, 1.6, 14, 1.6)
+    ang = torch.tensor((2 * math.pi * np.arange(20) / 20).astype(np.float32))
+    g = torch.Generator().manual_seed(4)
+    truth = torch.rand(vol.shape_zyx, generator=g).to(DEV)
+    b = nh.project(truth, vol, det, ang) + 0.01 * torch.randn((20, 14, 16), generator=g).to(DEV)
+    mu = 0.05
+    n = truth.numel()
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = torch.zeros(n, device=DEV)
+        e[j] = 1.0
+        e = e.reshape(vol.shape_zyx)
+        col = nh.backproject(nh.project(e, vol, det, ang), vol, det, ang) + mu * tv_diff_adjoint(tv_diff(e))
+        M[:, j] = col.reshape(-1).double().cpu().numpy()
+    rhs = nh.backproject(b, vol, det, ang).reshape(-1).double().cpu().numpy()
+    want = np.linalg.solve(0.5 * (M + M.T), rhs)
+    x = conj_grad(b, vol, det, ang, n_iter=120, mu=mu)
+    got = x.reshape(-1).double().cpu().numpy()
+    assert np.linalg.norm(got - want) <= 2e-3 * np.linalg.norm(want)
