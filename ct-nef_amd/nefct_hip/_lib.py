@@ -57,6 +57,9 @@ SIGNATURES = {
     "nefct_pixel_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp],
     "nefct_fdk_footprint_backproject_flat": [_vp, _vp, _vp, _vp, _i, _f, _f, _f, _f, _i, _f, _f, _i, _vp, _vp],
     "nefct_fdk_footprint_backproject_cyli": [_vp, _vp, _vp, _vp, _i, _f, _f, _f, _f, _i, _f, _f, _i, _vp, _vp],
+    "nefct_tv_smooth": [_vp, _vp, _f, _f, _vp, _vp, _vp],
+    "nefct_sq_norm": [_vp, _sz, ctypes.c_double, _vp, _vp],
+    "nefct_tv_descent_step": [_vp, _vp, _vp, _f, _f, _f, _vp, _vp],
     "nefct_batch_plan": [_vp, _i, _u, _vp, _vp, _vp],
     "nefct_batch_project": [_vp, _vp, _i, ctypes.c_longlong, _u, _vp, _sz, _vp],
     "nefct_batch_backproject": [_vp, _vp, _i, ctypes.c_longlong, _u, _vp, _sz, _vp],

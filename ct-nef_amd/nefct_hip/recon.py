@@ -16,10 +16,12 @@ adjoint (:125-149).  Here the same pieces run on the device:
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import torch
 
+from . import _lib
 from .geometry import ConeGeometry, VolumeGeometry
 from .siddon import backproject, forward_project, project
 
@@ -46,9 +48,74 @@ def tv_diff_adjoint(d: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def tv_smooth(x: torch.Tensor, eps: float = 1e-6) -> torch.Tensor:
+def tv_smooth_torch(x: torch.Tensor, eps: float = 1e-6) -> torch.Tensor:
+    """sum sqrt(d^2 + eps^2) over tv_diff(x), in torch ops (any device)."""
     d = tv_diff(x)
     return torch.sqrt(d * d + eps * eps).sum(dtype=torch.float64)
+
+
+def _sq_norm_into(x: torch.Tensor, value: torch.Tensor, scale: float = 1.0):
+    """value (FP64 device scalar) += scale * sum x^2, one kernel."""
+    from .ops import _ptr, _stream_ptr
+    x = x.contiguous()
+    st = _lib.load().nefct_sq_norm(_ptr(x), x.numel(), float(scale), _ptr(value),
+                                   ctypes.c_void_p(_stream_ptr(x.device)))
+    _lib.check(st, "sq_norm")
+
+
+def _tv_hip(x: torch.Tensor, eps: float, scale: float, want_value: bool, want_grad: bool, value=None):
+    from .ops import _need_device, _ptr, _stream_ptr
+    _need_device(x)
+    L = _lib.load()
+    x = x.contiguous()
+    nz, ny, nx = x.shape
+    grad = torch.empty_like(x) if want_grad else None
+    if want_value and value is None:
+        value = torch.zeros((), dtype=torch.float64, device=x.device)
+    st = L.nefct_tv_smooth(_ptr(x), _lib.int3((nx, ny, nz)), float(eps), float(scale),
+                           _ptr(grad) if grad is not None else None, _ptr(value) if value is not None else None,
+                           ctypes.c_void_p(_stream_ptr(x.device)))
+    _lib.check(st, "tv_smooth")
+    return value, grad
+
+
+class TVSmooth(torch.autograd.Function):
+    """TV(x) = sum sqrt(d^2 + eps^2) with the HIP value/gradient kernel."""
+
+    @staticmethod
+    def forward(ctx, x, eps):
+        ctx.save_for_backward(x)
+        ctx.eps = eps
+        value, _ = _tv_hip(x, eps, 1.0, True, False)
+        return value
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        _, grad = _tv_hip(x, ctx.eps, 1.0, False, True)
+        return (grad * g.to(grad.dtype)), None
+
+
+def tv_smooth(x: torch.Tensor, eps: float = 1e-6) -> torch.Tensor:
+    """Smoothed TV of a [nz, ny, nx] volume (FP64 value): the HIP kernel on a
+    device tensor, torch ops elsewhere."""
+    if x.is_cuda and x.dtype == torch.float32 and x.dim() == 3:
+        return TVSmooth.apply(x, float(eps))
+    return tv_smooth_torch(x, eps)
+
+
+def tv_descent_step(x: torch.Tensor, data_grad: torch.Tensor, eps: float, lam: float, step: float) -> torch.Tensor:
+    """x - step * (data_grad + lam * grad TV(x)) in one kernel."""
+    from .ops import _need_device, _ptr, _stream_ptr
+    _need_device(x, data_grad)
+    x = x.contiguous()
+    data_grad = data_grad.contiguous()
+    out = torch.empty_like(x)
+    nz, ny, nx = x.shape
+    st = _lib.load().nefct_tv_descent_step(_ptr(x), _ptr(data_grad), _lib.int3((nx, ny, nz)), float(eps),
+                                           float(lam), float(step), _ptr(out), ctypes.c_void_p(_stream_ptr(x.device)))
+    _lib.check(st, "tv_descent_step")
+    return out
 
 
 def loss(x, b, vol: VolumeGeometry, det: ConeGeometry, angles, lam: float = 1e-3, eps: float = 1e-6):
@@ -71,24 +138,66 @@ def power_iteration(vol, det, angles, iters: int = 10, device="cuda", seed: int 
 
 
 def gradient_descent(b, vol, det, angles, steps: int = 50, lam: float = 1e-3, eps: float = 1e-6, x0=None,
-                     lipschitz: float = None, callback=None):
-    """x_{k+1} = x_k - grad f(x_k) / L from a zero start (or x0)."""
+                     lipschitz: float = None, callback=None, fused: bool = True, graph: bool = False):
+    """x_{k+1} = x_k - grad f(x_k) / L from a zero start (or x0), f the
+    ``loss``.  ``fused`` (device tensors): project, backproject, then one
+    kernel for TV gradient + update; otherwise the autograd loss.  ``graph``
+    (with ``fused``) captures one step in a HIP graph and replays it.
+    Returns (x, loss history)."""
     dev = b.device
     x = torch.zeros(vol.shape_zyx, dtype=torch.float32, device=dev) if x0 is None else x0.clone()
     if lipschitz is None:
         lipschitz = power_iteration(vol, det, angles, device=dev)
     L = lipschitz * 1.05 + (12.0 * lam / eps if lam else 0.0)
     history = []
+    if not (fused and b.is_cuda):
+        for k in range(steps):
+            x.requires_grad_(True)
+            f = loss(x, b, vol, det, angles, lam, eps)
+            (g,) = torch.autograd.grad(f, x)
+            with torch.no_grad():
+                x = (x - g / L).detach()
+            history.append(float(f.detach()))
+            if callback:
+                callback(k, x, float(f.detach()))
+        return x, history
+
+    ang = torch.as_tensor(angles, dtype=torch.float32, device=dev)
+    tv_eps = eps if lam else 1.0
+    fval = torch.zeros((), dtype=torch.float64, device=dev)
+
+    def step():
+        # no torch reductions here: the loss value is summed by this
+        # library's kernels, so the step can be captured in a graph
+        r = project(x, vol, det, ang) - b
+        g = backproject(r, vol, det, ang)
+        fval.zero_()
+        _sq_norm_into(r, fval, 0.5)
+        if lam:
+            _tv_hip(x, tv_eps, lam, True, False, value=fval)
+        x.copy_(tv_descent_step(x, g, tv_eps, lam, 1.0 / L))
+
+    if graph:
+        s = torch.cuda.Stream(device=dev)
+        s.wait_stream(torch.cuda.current_stream(dev))
+        x_start = x.clone()
+        with torch.cuda.stream(s):
+            step()                       # warm-up outside the graph
+        torch.cuda.current_stream(dev).wait_stream(s)
+        x.copy_(x_start)
+        g_ = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_):
+            step()
+        x.copy_(x_start)
+        run = g_.replay
+    else:
+        run = step
     for k in range(steps):
-        x.requires_grad_(True)
-        f = loss(x, b, vol, det, angles, lam, eps)
-        (g,) = torch.autograd.grad(f, x)
-        with torch.no_grad():
-            x = (x - g / L).detach()
-        history.append(float(f))
+        run()
+        history.append(float(fval))
         if callback:
-            callback(k, x, float(f))
-    return x, history
+            callback(k, x, history[-1])
+    return x.clone(), history
 
 
 def sart(b, vol, det, angles, iters: int = 10, relax: float = 1.0, x0=None):

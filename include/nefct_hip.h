@@ -264,6 +264,28 @@ int nefct_fdk_footprint_backproject_cyli(const float* pv_values, const int shape
                                          const float* angles, int nv, float SID, float SAD, float da, float ai,
                                          int na, float db, float bi, int nb, float* image, void* stream);
 
+/* Smoothed anisotropic total variation TV(x) = sum_k sum_v sqrt(d_k(v)^2 +
+ * eps^2) over the forward differences with a zero first slice of
+ * nefct/functions/total_variation.py:19-35 (whose transpose is
+ * InvertTotalVari, :125-149).  grad (may be NULL) = scale * dTV/dx, fully
+ * written; *value (may be NULL; device FP64) += scale * TV(x).  image [nz][ny][nx].
+ * No reference counterpart for the smoothing (the reference's TV is used
+ * inside ADMM, conj_grad.py:20-24). */
+int nefct_tv_smooth(const float* image, const int grid[3], float eps, float scale, float* grad, double* value,
+                    void* stream);
+
+/* *value (device FP64) += scale * sum x[i]^2, accumulated in FP64; the
+ * caller clears *value first.  For loss values inside captured graphs. */
+int nefct_sq_norm(const float* x, size_t n, double scale, double* value, void* stream);
+
+/* One fused gradient-descent update of 0.5 ||A x - b||^2 + lam TV(x):
+ * out = image - step * (data_grad + lam * dTV/dx(image)), data_grad =
+ * A^T (A x - b) from the adjoint.  out != image.  No reference counterpart
+ * (the reference iterates in numpy, nefct/functions/sart.py:29-39,
+ * conj_grad.py:50-62). */
+int nefct_tv_descent_step(const float* image, const float* data_grad, const int grid[3], float eps, float lam,
+                          float step, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

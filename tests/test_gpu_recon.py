@@ -78,3 +78,33 @@ def test_conj_grad_approaches_the_dense_normal_equation_solution():
     x = conj_grad(b, vol, det, ang, n_iter=120, mu=mu)
     got = x.reshape(-1).double().cpu().numpy()
     assert np.linalg.norm(got - want) <= 2e-3 * np.linalg.norm(want)
+
+
+def test_tv_kernel_matches_torch_value_and_gradient():
+    from nefct_hip.recon import TVSmooth, tv_smooth_torch
+    g = torch.Generator().manual_seed(8)
+    x = torch.rand((13, 17, 70), generator=g).to(DEV)
+    x[4:8, 3:9, 10:40] += 1.0
+    for eps in (1e-3, 0.1):
+        xa = x.clone().requires_grad_(True)
+        va = TVSmooth.apply(xa, eps)
+        va.backward()
+        xb = x.clone().double().requires_grad_(True)
+        vb = tv_smooth_torch(xb, eps)
+        vb.backward()
+        assert va.item() == pytest.approx(vb.item(), rel=1e-6)
+        assert torch.allclose(xa.grad.double(), xb.grad, rtol=1e-4, atol=1e-5)
+
+
+def test_fused_and_graph_descent_match_autograd_descent():
+    from nefct_hip.recon import gradient_descent, power_iteration
+    vol, det, ang, truth, b = _setup(24, 20)
+    Lp = power_iteration(vol, det, ang, device=DEV)
+    xa, ha = gradient_descent(b, vol, det, ang, steps=5, lam=1e-3, eps=1e-2, lipschitz=Lp, fused=False)
+    xf, hf = gradient_descent(b, vol, det, ang, steps=5, lam=1e-3, eps=1e-2, lipschitz=Lp, fused=True)
+    xg, hg = gradient_descent(b, vol, det, ang, steps=5, lam=1e-3, eps=1e-2, lipschitz=Lp, fused=True, graph=True)
+    assert np.allclose(ha, hf, rtol=1e-5)
+    assert np.allclose(hf, hg, rtol=1e-6)
+    scale = xa.abs().max().item()
+    assert torch.allclose(xf, xa, rtol=1e-4, atol=1e-5 * scale)
+    assert torch.allclose(xg, xf, rtol=1e-5, atol=1e-6 * scale)
