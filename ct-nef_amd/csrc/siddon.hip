@@ -796,9 +796,62 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
             const float sv = pv[id] * r.scale;
             const Walk w = walk_of(m, P.V, P.Ly);
+            // LDS word of centred transverse cell (ca, cb) at slab t:
+            // ca Sa + cb Sb + t St + base, with the strides of the walk axis;
+            // [t0, t1) lies inside the block, so only ca and cb are range-checked
+            constexpr int SXY = TY * SX;
+            const int Sa = (m == 0) ? SX : 1;
+            const int Sb = (m == 2) ? SX : SXY;
+            const int St = (m == 0) ? 1 : ((m == 1) ? SX : SXY);
+            const int la = b0[a] - ha, lb = b0[b] - hb;     // centred cell of the block's first a / b index
+            const unsigned ea = (unsigned)ext[a], eb = (unsigned)ext[b];
+            int row = (t0 - b0[m]) * St - la * Sa - lb * Sb;
             float tf = (float)(t0 - r.tc);
             float a0 = fmaf(r.ka, tf, r.Ac), bb0 = fmaf(r.kb, tf, r.Bc);
             float fla0 = floorf(a0), flb0 = floorf(bb0);
+            if (!REF) {
+                // Exact rule with the slab's entry cell carried from the previous
+                // slab: its integer index, LDS offset and range test are computed
+                // once per slab, not once per segment.  Same arithmetic as
+                // slab_segments<false>.
+                // byte offsets into the block's LDS array
+                char* const ab = reinterpret_cast<char*>(af);
+                const int Sa4 = 4 * Sa, Sb4 = 4 * Sb, St4 = 4 * St;
+                int row4 = 4 * row;
+                int ca0 = (int)fla0, cb0 = (int)flb0;
+                int A0 = __mul24(ca0, Sa4), B0 = __mul24(cb0, Sb4);
+                bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
+#pragma unroll 2
+                for (int t = t0; t < t1; ++t) {
+                    tf += 1.0f;
+                    const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
+                    const float fla1 = floorf(a1), flb1 = floorf(bb1);
+                    const int ca1 = (int)fla1, cb1 = (int)flb1;
+                    const float fa = (ca1 != ca0) ? clamp01((fla0 + r.pa - a0) * r.inv_ka) : 1.0f;
+                    const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - bb0) * r.inv_kb) : 1.0f;
+                    const bool afirst = fa < fb;
+                    const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
+                    const int A1 = __mul24(ca1, Sa4), B1 = __mul24(cb1, Sb4);
+                    const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
+                    const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
+                    if (ia0 && ib0 && w1 > 0.0f) atomicAdd((float*)(ab + (A0 + B0 + row4)), w1 * sv);
+                    const bool i2 = afirst ? (ia1 && ib0) : (ia0 && ib1);
+                    if (i2 && w2 > 0.0f) atomicAdd((float*)(ab + ((afirst ? A1 + B0 : A0 + B1) + row4)), w2 * sv);
+                    if (ia1 && ib1 && w3 > 0.0f) atomicAdd((float*)(ab + (A1 + B1 + row4)), w3 * sv);
+                    a0 = a1;
+                    bb0 = bb1;
+                    fla0 = fla1;
+                    flb0 = flb1;
+                    ca0 = ca1;
+                    cb0 = cb1;
+                    A0 = A1;
+                    B0 = B1;
+                    ia0 = ia1;
+                    ib0 = ib1;
+                    row4 += St4;
+                }
+                continue;
+            }
             for (int t = t0; t < t1; ++t) {
                 tf += 1.0f;
                 const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
@@ -811,20 +864,15 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     const float wk[3] = {sg.w1, sg.w2, sg.w3};
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        if (!(wk[q] > 0.0f)) continue;
-                        int l[3];
-                        l[m] = t - b0[m];
-                        l[a] = ca[q] + ha - b0[a];
-                        l[b] = cb[q] + hb - b0[b];
-                        if ((unsigned)l[0] < (unsigned)ext[0] && (unsigned)l[1] < (unsigned)ext[1] &&
-                            (unsigned)l[2] < (unsigned)ext[2])
-                            atomicAdd(&acc.v[l[2]][l[1]][l[0]], wk[q] * sv);
+                        if (wk[q] > 0.0f && (unsigned)(ca[q] - la) < ea && (unsigned)(cb[q] - lb) < eb)
+                            atomicAdd(af + (ca[q] * Sa + cb[q] * Sb + row), wk[q] * sv);
                     }
                 }
                 a0 = a1;
                 bb0 = bb1;
                 fla0 = fla1;
                 flb0 = flb1;
+                row += St;
             }
         }
     }
