@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain leg")
     ap.add_argument("--batched-iters", type=int, default=2)
+    ap.add_argument("--calib", action="store_true",
+                    help="profiling only: first copy 256 MiB device-to-device (a known-size dispatch that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -165,6 +168,13 @@ def main():
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(dev, args.batched_iters)}), flush=True)
         return
+    if args.calib:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] calibration copy: 268435456 bytes read, 268435456 bytes written")
     log(f"[bench] workload {wl.name}: building phantom")
     img = ellipsoid_phantom(vol, wl.seed, wl.ellipsoids, device=dev)
     ang = torch.tensor(wl.angles(), device=dev)

@@ -1189,6 +1189,12 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
             } else if (shape == NEFCT_F_TILE_16x16x32) {
                 if (cs2) launch_tile<16, 16, 32, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
                 else launch_tile<16, 16, 32, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
+            } else if (shape == NEFCT_F_TILE_64x64x8) {
+                if (cs2) launch_tile<64, 64, 8, 16, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<64, 64, 8, 16, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
+            } else if (shape == NEFCT_F_TILE_64x32x16) {
+                if (cs2) launch_tile<64, 32, 16, 16, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
+                else launch_tile<64, 32, 16, 16, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
             } else {
                 if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
                 else launch_tile<32, 32, 8, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);

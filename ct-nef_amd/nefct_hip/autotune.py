@@ -2,7 +2,7 @@
 
 The exact adjoint has several implementations with identical weights that
 differ in how work maps onto the chip (include/nefct_hip.h, adjoint flags):
-volume blocks of 32x32x8, 32x32x12, 64x16x8 or 16x16x32 voxels, neighbouring
+volume blocks of 32x32x8, 32x32x12, 64x16x8, 16x16x32, 64x64x8 or 64x32x16 voxels, neighbouring
 lanes one or two detector columns apart, per-ray parameters read from a
 table or recomputed, or ray tiles with LDS slab windows.  Which is fastest
 depends on the volume, detector and view count, so the tuner times the
@@ -34,6 +34,9 @@ ADJOINT_CANDIDATES: Dict[str, int] = {
     "tile64x16x8": _lib.F_TILE_64x16x8,
     "tile64x16x8/stride2": _lib.F_TILE_64x16x8 | _lib.F_LANE_STRIDE2,
     "tile16x16x32": _lib.F_TILE_16x16x32,
+    "tile64x64x8": _lib.F_TILE_64x64x8,
+    "tile64x64x8/stride2": _lib.F_TILE_64x64x8 | _lib.F_LANE_STRIDE2,
+    "tile64x32x16": _lib.F_TILE_64x32x16,
     "window": _lib.F_ADJ_WINDOW,
 }
 

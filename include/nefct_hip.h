@@ -62,15 +62,18 @@ typedef enum {
 #define NEFCT_F_DETERMINISTIC 16u
 #define NEFCT_F_ADJ_WINDOW 32u
 #define NEFCT_F_ADJ_RECOMPUTE 64u
-/* Block shape of the volume-tile adjoint (x by y by z voxels, 4 wavefronts)
- * and the spacing of a wavefront's rays; the autotuner picks among these.
- * The deterministic variant always uses 16x16x8 blocks of one wavefront. */
-#define NEFCT_F_TILE_MASK 0x300u
+/* Block shape of the volume-tile adjoint (x by y by z voxels; 4 wavefronts,
+ * 16 for the large blocks, whose LDS array is 133 KB) and the spacing of a
+ * wavefront's rays; the autotuner picks among these.  The deterministic
+ * variant always uses 16x16x8 blocks of one wavefront. */
+#define NEFCT_F_TILE_MASK 0x700u
 #define NEFCT_F_TILE_32x32x8 0x000u  /* default */
 #define NEFCT_F_TILE_32x32x12 0x100u
 #define NEFCT_F_TILE_64x16x8 0x200u
 #define NEFCT_F_TILE_16x16x32 0x300u
-#define NEFCT_F_LANE_STRIDE2 0x400u  /* neighbouring lanes take every other detector column */
+#define NEFCT_F_TILE_64x64x8 0x400u
+#define NEFCT_F_TILE_64x32x16 0x500u
+#define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector column */
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
 

@@ -227,7 +227,9 @@ def test_autograd_matches_adjoint():
     assert torch.allclose(x.grad, ref, rtol=1e-5, atol=1e-6 * ref.abs().max().item())
 
 
-@pytest.mark.parametrize("algo", ["tile", "recompute", "deterministic", "deterministic-recompute", "window"])
+@pytest.mark.parametrize("algo", ["tile", "recompute", "deterministic", "deterministic-recompute", "window",
+                                  "tile32x32x12-stride2", "tile64x16x8", "tile16x16x32", "tile64x64x8",
+                                  "tile64x32x16-stride2", "tile64x64x8-recompute"])
 def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
     """Every adjoint variant against the FP64 transpose.  At these sizes the
     per-ray table of the tile variants holds 4-6 views, so the table path runs
@@ -236,7 +238,12 @@ def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
     from nefct_hip import _lib
     fl = {"tile": 0, "recompute": _lib.F_ADJ_RECOMPUTE, "deterministic": _lib.F_DETERMINISTIC,
           "deterministic-recompute": _lib.F_DETERMINISTIC | _lib.F_ADJ_RECOMPUTE,
-          "window": _lib.F_ADJ_WINDOW}[algo]
+          "window": _lib.F_ADJ_WINDOW,
+          "tile32x32x12-stride2": _lib.F_TILE_32x32x12 | _lib.F_LANE_STRIDE2,
+          "tile64x16x8": _lib.F_TILE_64x16x8, "tile16x16x32": _lib.F_TILE_16x16x32,
+          "tile64x64x8": _lib.F_TILE_64x64x8,
+          "tile64x32x16-stride2": _lib.F_TILE_64x32x16 | _lib.F_LANE_STRIDE2,
+          "tile64x64x8-recompute": _lib.F_TILE_64x64x8 | _lib.F_ADJ_RECOMPUTE}[algo]
     for gi in (0, 1, 2):
         name, vol, det = _geoms()[gi]
         _, ang, img, ov, od = _case(oracle, vol, det, 16, seed=gi + 21)
