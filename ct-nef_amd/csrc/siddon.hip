@@ -1316,7 +1316,7 @@ __global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __re
 }
 
 template <bool REF>
-__global__ __launch_bounds__(256) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
+__global__ __launch_bounds__(256, 3) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                    long long total, char* ws_base, unsigned flags)
 {
     __shared__ TileAcc<BT, BT, BTZ> acc;
@@ -1513,7 +1513,7 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
     batch_prep(host, dev, nprob, ws, flags, s);
-    const int g = persistent_grid(4);
+    const int g = persistent_grid(3);   // 3 workgroups of 4 waves per CU fit (168 VGPRs)
     if (flags & NEFCT_F_REF_SLAB)
         hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
     else
