@@ -76,6 +76,17 @@ struct Ray {
 
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
+// floor / ceil of v as an int clamped to [lo, hi] before the conversion, so
+// huge or non-finite values (slopes near zero) cannot overflow; NaN gives lo
+__device__ __forceinline__ int floor_clamp(float v, int lo, int hi)
+{
+    return (int)fminf(fmaxf(floorf(v), (float)lo), (float)hi);
+}
+__device__ __forceinline__ int ceil_clamp(float v, int lo, int hi)
+{
+    return (int)fminf(fmaxf(ceilf(v), (float)lo), (float)hi);
+}
+
 __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, int ia, int ib,
                                            double S[3], double D[3])
 {
@@ -717,10 +728,10 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
         ia0 = 0; ia1 = G.na - 1; ib0 = 0; ib1 = G.nb - 1;
         return;
     }
-    ia0 = __builtin_amdgcn_readfirstlane(max(0, (int)floorf(fa_mn) - 1));
-    ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, (int)ceilf(fa_mx) + 1));
-    ib0 = __builtin_amdgcn_readfirstlane(max(0, (int)floorf(fb_mn) - 1));
-    ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, (int)ceilf(fb_mx) + 1));
+    ia0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fa_mn, -2, G.na + 1) - 1));
+    ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, ceil_clamp(fa_mx, -2, G.na + 1) + 1));
+    ib0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fb_mn, -2, G.nb + 1) - 1));
+    ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, ceil_clamp(fb_mx, -2, G.nb + 1) + 1));
 }
 
 template <bool REF, int TX, int TY, int TZ, int NW, int CS = 1, bool TABLE = false>
@@ -753,14 +764,40 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         const int wv = tid >> 6, ln = tid & 63;
         const int cols = CS * ((ia1 - ia0 + 64 * CS) / (64 * CS));
         const int nk = cols * ((ib1 - ib0 + NW) / NW);
-        for (int k = 0; k < nk; ++k) {
+        // candidate k -> detector cell; the next candidate's projection value
+        // (and table record) is loaded before the current ray is walked, so
+        // the loads' latency overlaps the walk
+        auto cell_of = [&](int k, int& ia, int& ib) {
             const int kc = k % cols;
-            const int ib = ib0 + wv + NW * (k / cols);
-            const int ia = ia0 + (kc / CS) * (64 * CS) + CS * ln + (kc % CS);
-            if (ib > ib1 || ia > ia1) continue;
+            ib = ib0 + wv + NW * (k / cols);
+            ia = ia0 + (kc / CS) * (64 * CS) + CS * ln + (kc % CS);
+            return ib <= ib1 && ia <= ia1;
+        };
+        const size_t view_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
+        const size_t rec_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)(iv - v0);
+        int ia_n = 0, ib_n = 0;
+        bool ok_n = nk > 0 && cell_of(0, ia_n, ib_n);
+        float pv_n = 0.0f;
+        RayRec q_n;
+        if (ok_n) {
+            const size_t o = (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n;
+            pv_n = pv[view_base + o];
+            if (TABLE) q_n = table[rec_base + o];
+        }
+        for (int k = 0; k < nk; ++k) {
+            const int ia = ia_n, ib = ib_n;
+            const bool ok = ok_n;
+            const float pval = pv_n;
+            const RayRec q = q_n;
+            ok_n = k + 1 < nk && cell_of(k + 1, ia_n, ib_n);
+            if (ok_n) {
+                const size_t o = (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n;
+                pv_n = pv[view_base + o];
+                if (TABLE) q_n = table[rec_base + o];
+            }
+            if (!ok) continue;
             Ray r;
             if (TABLE) {
-                const RayRec q = table[((size_t)(iv - v0) * P.G.nb + ib) * P.G.na + ia];
                 unpack_ray(q, r);
             } else {
                 setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
@@ -780,21 +817,22 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             if (fmaxf(fa0, fa1) < alo || fminf(fa0, fa1) > ahi || fmaxf(fb0, fb1) < blo || fminf(fb0, fb1) > bhi)
                 continue;
             // tighten [t0, t1) to the slabs where the ray is within the margin box
+            // (the clamps keep the result while bounding the conversion:
+            // t0 and t1 already lie in [b0[m], b1[m]])
             if (r.ka != 0.0f) {
                 float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, (int)floorf(ta) - 1);
-                t1 = min(t1, (int)ceilf(tb) + 1);
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
             }
             if (r.kb != 0.0f) {
                 float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, (int)floorf(ta) - 1);
-                t1 = min(t1, (int)ceilf(tb) + 1);
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
             }
             if (t0 >= t1) continue;
-            const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
-            const float sv = pv[id] * r.scale;
+            const float sv = pval * r.scale;
             const Walk w = walk_of(m, P.V, P.Ly);
             // LDS word of centred transverse cell (ca, cb) at slab t:
             // ca Sa + cb Sb + t St + base, with the strides of the walk axis;
@@ -887,7 +925,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
 }
 
 template <bool REF, int TX, int TY, int TZ, int NW, int CS, bool TABLE>
-__global__ __launch_bounds__(64 * NW) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
+__global__ __launch_bounds__(64 * NW, 4) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
                                                              float* __restrict__ image, const RayRec* __restrict__ table,
                                                              int v0, int v1, int accumulate)
 {
