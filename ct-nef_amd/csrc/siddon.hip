@@ -20,6 +20,7 @@
 //  * the adjoint scatters with buffer float atomics into the same two
 //    layouts, then one pass sums them into the x-fastest output.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <initializer_list>
 #include <math.h>
@@ -340,7 +341,32 @@ struct Launch {
     const double* trig;  // (cos, sin) per view
     int nv;
     unsigned flags;
+    float sbound;        // upper bound of any ray's mm per slab (fixed-point adjoint scale)
 };
+
+// Upper bound of Ray::scale over every ray of the geometry (mm of ray per
+// slab, times 1/L^2 under INV_L2), with 1 % margin.  Exact mode: the walk axis
+// is the ray's largest component in voxel units, so the length per slab is at
+// most the voxel diagonal.  Reference mode walks x or y: the in-plane part is
+// at least 1/sqrt(2) of the in-plane length, and the source-detector vector
+// rises at most |b|max over an in-plane length of at least SAD.
+__host__ __device__ inline float scale_bound(const VolDev& V, const DetDev& G, unsigned flags)
+{
+    double bound;
+    if (!(flags & NEFCT_F_REF_SLAB)) {
+        bound = sqrt(V.vs[0] * V.vs[0] + V.vs[1] * V.vs[1] + V.vs[2] * V.vs[2]);
+    } else {
+        const double b_lo = G.bi * G.db - G.nb * G.db * 0.5, b_hi = (G.bi + G.nb - 1) * G.db - G.nb * G.db * 0.5;
+        const double bmax = fmax(fabs(b_lo), fabs(b_hi)) + G.db;
+        const double q = bmax / G.sad;
+        bound = fmax(V.vs[0], V.vs[1]) * sqrt(2.0) * sqrt(1.0 + q * q);
+    }
+    if (flags & NEFCT_F_INV_L2) {
+        const double lmin = G.sad * V.inv_vs[0];   // shortest source-detector distance, x-voxel units
+        if (lmin < 1.0) bound /= lmin * lmin;
+    }
+    return (float)(bound * 1.01);
+}
 
 template <int MODE, bool REF>
 __device__ float project_all_dominances(const Launch& P, const Ray& r, bool inside, float val)
@@ -614,62 +640,6 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
 }
 
 // ---------------------------------------------------------------------------
-// Per-ray parameter table: the FP64 set-up of every ray of a chunk of views,
-// packed into 32 bytes, so that consumers that visit a ray many times (the
-// volume-tile adjoint) read the forward's exact FP32 parameters instead of
-// recomputing them.  tc is (s0 + s1) >> 1 and the slope signs come from
-// ka, kb, so 8 words suffice.
-// ---------------------------------------------------------------------------
-struct RayRec {
-    float4 p0;  // Ac, ka, Bc, kb
-    float4 p1;  // inv_ka, inv_kb, scale, bits(s0 | s1 << 15 | dom << 30)
-};
-
-__device__ __forceinline__ RayRec pack_ray(const Ray& r)
-{
-    RayRec q;
-    q.p0 = make_float4(r.Ac, r.ka, r.Bc, r.kb);
-    const unsigned dom = (r.dom < 0) ? 3u : (unsigned)r.dom;
-    const unsigned bits = (unsigned)r.s0 | ((unsigned)r.s1 << 15) | (dom << 30);
-    q.p1 = make_float4(r.inv_ka, r.inv_kb, r.scale, __uint_as_float(bits));
-    return q;
-}
-
-__device__ __forceinline__ void unpack_ray(const RayRec& q, Ray& r)
-{
-    r.Ac = q.p0.x;
-    r.ka = q.p0.y;
-    r.Bc = q.p0.z;
-    r.kb = q.p0.w;
-    r.inv_ka = q.p1.x;
-    r.inv_kb = q.p1.y;
-    r.scale = q.p1.z;
-    const unsigned bits = __float_as_uint(q.p1.w);
-    r.s0 = (int)(bits & 0x7fffu);
-    r.s1 = (int)((bits >> 15) & 0x7fffu);
-    const unsigned dom = bits >> 30;
-    r.dom = (dom == 3u) ? -1 : (int)dom;
-    r.tc = (r.s0 + r.s1) >> 1;
-    // ka and 1/ka share their sign; one of them is a normal float whenever
-    // the slope is non-zero, so the test below matches the FP64 one.
-    r.pa = (r.ka > 0.0f || r.inv_ka > 0.0f) ? 1.0f : 0.0f;
-    r.pb = (r.kb > 0.0f || r.inv_kb > 0.0f) ? 1.0f : 0.0f;
-}
-
-// table[(iv - v0) * nb * na + ib * na + ia] for views [v0, v0 + nvc)
-__global__ __launch_bounds__(256) void k_ray_table(Launch P, int v0, RayRec* __restrict__ table, int adjoint)
-{
-    const int ia = blockIdx.x * 256 + threadIdx.x;
-    const int ib = blockIdx.y;
-    const int iv = v0 + blockIdx.z;
-    if (ia >= P.G.na) return;
-    const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
-    Ray r;
-    setup_ray(P.V, P.G, c, s, ia, ib, P.flags, adjoint != 0, r);
-    table[((size_t)blockIdx.z * P.G.nb + ib) * P.G.na + ia] = pack_ray(r);
-}
-
-// ---------------------------------------------------------------------------
 // Adjoint owned by volume tiles (no global atomics).
 //
 // A workgroup owns a TX x TY x TZ block of voxels held in LDS.  For every view
@@ -680,14 +650,50 @@ __global__ __launch_bounds__(256) void k_ray_table(Launch P, int v0, RayRec* __r
 // the last view the block is written once.  Every (ray, slab, cell) weight is
 // the forward kernel's own arithmetic, and every voxel belongs to one block,
 // so the result is the transpose of the forward up to summation order.
-// With one wavefront per block (NW = 1) the LDS summation order is fixed, so
-// the result is bitwise reproducible.
+//
+// Accumulation (FIX = true, the default): each contribution w * pv * scale
+// is rounded to a signed fixed-point integer with a power-of-two scale S and
+// added to a 64-bit integer in LDS (ds_add_u64).  S is the largest power of
+// two with |contribution| * S <= 2^30 for every contribution of the call
+// (from max |pv| over the sinogram and a geometric bound on the per-slab ray
+// length), so the rounding step is ~1e-9 of the largest contribution, and
+// integer sums do not depend on the order of the additions: the result is
+// bitwise reproducible.  LDS float atomics (ds_add_f32, FIX = false) measure
+// several times slower than integer ones on gfx950, which is why this exists.
+// A non-finite projection value makes the affected blocks NaN.
 // ---------------------------------------------------------------------------
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, typename T>
 struct TileAcc {
     static constexpr int SX = TX + 1;  // odd row stride: lanes on consecutive y rows hit distinct banks
-    float v[TZ][TY][SX];
+    T v[TZ][TY][SX];
 };
+
+template <bool FIX>
+using acc_t = typename std::conditional<FIX, unsigned long long, float>::type;
+
+// Power-of-two fixed-point scale from the sinogram's max |value| (as float
+// bits) and the bound on the per-slab ray length; 0 marks non-finite input.
+__device__ __forceinline__ float fixed_scale(unsigned maxbits, float sbound)
+{
+    const float mx = __uint_as_float(maxbits);
+    if (!(mx <= 3.0e38f)) return 0.0f;          // Inf or NaN somewhere
+    const float big = mx * sbound;
+    if (!(big > 0.0f)) return 1.0f;             // all zero
+    int e;
+    frexpf(1073741824.0f / big, &e);            // 2^30 / big = m 2^e, m in [0.5, 1)
+    return ldexpf(1.0f, e - 1);                 // largest power of two <= 2^30 / big
+}
+
+template <bool FIX>
+__device__ __forceinline__ void lds_deposit(char* base, int byte_off, float v, float S)
+{
+    if (FIX) {
+        const long long q = (long long)(int)rintf(v * S);
+        atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), (unsigned long long)q);
+    } else {
+        atomicAdd(reinterpret_cast<float*>(base + byte_off), v);
+    }
+}
 
 __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
                                               const int hi[3], int& ia0, int& ia1, int& ib0, int& ib1)
@@ -734,24 +740,24 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
     ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, ceil_clamp(fb_mx, -2, G.nb + 1) + 1));
 }
 
-template <bool REF, int TX, int TY, int TZ, int NW, int CS = 1, bool TABLE = false>
+template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
-                                               int bx, int by, int bz, TileAcc<TX, TY, TZ>& acc,
-                                               const RayRec* __restrict__ table = nullptr, int v0 = 0, int v1 = -1,
-                                               bool accumulate = false)
+                                               int bx, int by, int bz, TileAcc<TX, TY, TZ, acc_t<FIX>>& acc,
+                                               float S, bool accumulate)
 {
-    constexpr int SX = TileAcc<TX, TY, TZ>::SX;
+    using T = acc_t<FIX>;
+    constexpr int SX = TileAcc<TX, TY, TZ, T>::SX;
+    constexpr int ES = (int)sizeof(T);
     const int tid = threadIdx.x;
-    float* af = &acc.v[0][0][0];
+    T* af = &acc.v[0][0][0];
     __syncthreads();   // the previous block's write-out has read acc
-    for (int i = tid; i < TZ * TY * SX; i += 64 * NW) af[i] = 0.0f;
+    for (int i = tid; i < TZ * TY * SX; i += 64 * NW) af[i] = (T)0;
     const int nx = P.V.n[0], ny = P.V.n[1], nz = P.V.n[2];
     const int b0[3] = {bx * TX, by * TY, bz * TZ};
     const int ext[3] = {min(TX, nx - b0[0]), min(TY, ny - b0[1]), min(TZ, nz - b0[2])};
     const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
     __syncthreads();
-    if (v1 < 0) v1 = P.nv;
-    for (int iv = v0; iv < v1; ++iv) {
+    for (int iv = 0; iv < P.nv; ++iv) {
         const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
         int ia0, ia1, ib0, ib1;
         shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
@@ -765,8 +771,8 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         const int cols = CS * ((ia1 - ia0 + 64 * CS) / (64 * CS));
         const int nk = cols * ((ib1 - ib0 + NW) / NW);
         // candidate k -> detector cell; the next candidate's projection value
-        // (and table record) is loaded before the current ray is walked, so
-        // the loads' latency overlaps the walk
+        // is loaded before the current ray is walked, so the load's latency
+        // overlaps the walk
         auto cell_of = [&](int k, int& ia, int& ib) {
             const int kc = k % cols;
             ib = ib0 + wv + NW * (k / cols);
@@ -774,34 +780,18 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             return ib <= ib1 && ia <= ia1;
         };
         const size_t view_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
-        const size_t rec_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)(iv - v0);
         int ia_n = 0, ib_n = 0;
         bool ok_n = nk > 0 && cell_of(0, ia_n, ib_n);
-        float pv_n = 0.0f;
-        RayRec q_n;
-        if (ok_n) {
-            const size_t o = (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n;
-            pv_n = pv[view_base + o];
-            if (TABLE) q_n = table[rec_base + o];
-        }
+        float pv_n = ok_n ? pv[view_base + (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n] : 0.0f;
         for (int k = 0; k < nk; ++k) {
             const int ia = ia_n, ib = ib_n;
             const bool ok = ok_n;
             const float pval = pv_n;
-            const RayRec q = q_n;
             ok_n = k + 1 < nk && cell_of(k + 1, ia_n, ib_n);
-            if (ok_n) {
-                const size_t o = (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n;
-                pv_n = pv[view_base + o];
-                if (TABLE) q_n = table[rec_base + o];
-            }
+            if (ok_n) pv_n = pv[view_base + (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n];
             if (!ok) continue;
             Ray r;
-            if (TABLE) {
-                unpack_ray(q, r);
-            } else {
-                setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
-            }
+            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
             if (r.dom < 0) continue;
             const int m = r.dom, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
             // slabs of the block on the walk axis, within the ray's own range
@@ -834,15 +824,17 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             if (t0 >= t1) continue;
             const float sv = pval * r.scale;
             const Walk w = walk_of(m, P.V, P.Ly);
-            // LDS word of centred transverse cell (ca, cb) at slab t:
-            // ca Sa + cb Sb + t St + base, with the strides of the walk axis;
-            // [t0, t1) lies inside the block, so only ca and cb are range-checked
+            // LDS byte offset of centred transverse cell (ca, cb) at slab t:
+            // (ca Sa + cb Sb + t St + base) * ES, with the strides of the walk
+            // axis; [t0, t1) lies inside the block, so only ca and cb are
+            // range-checked
             constexpr int SXY = TY * SX;
-            const int Sa = (m == 0) ? SX : 1;
-            const int Sb = (m == 2) ? SX : SXY;
-            const int St = (m == 0) ? 1 : ((m == 1) ? SX : SXY);
+            const int Sa = ES * ((m == 0) ? SX : 1);
+            const int Sb = ES * ((m == 2) ? SX : SXY);
+            const int St = ES * ((m == 0) ? 1 : ((m == 1) ? SX : SXY));
             const int la = b0[a] - ha, lb = b0[b] - hb;     // centred cell of the block's first a / b index
             const unsigned ea = (unsigned)ext[a], eb = (unsigned)ext[b];
+            char* const ab = reinterpret_cast<char*>(af);
             int row = (t0 - b0[m]) * St - la * Sa - lb * Sb;
             float tf = (float)(t0 - r.tc);
             float a0 = fmaf(r.ka, tf, r.Ac), bb0 = fmaf(r.kb, tf, r.Bc);
@@ -852,12 +844,8 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 // slab: its integer index, LDS offset and range test are computed
                 // once per slab, not once per segment.  Same arithmetic as
                 // slab_segments<false>.
-                // byte offsets into the block's LDS array
-                char* const ab = reinterpret_cast<char*>(af);
-                const int Sa4 = 4 * Sa, Sb4 = 4 * Sb, St4 = 4 * St;
-                int row4 = 4 * row;
                 int ca0 = (int)fla0, cb0 = (int)flb0;
-                int A0 = __mul24(ca0, Sa4), B0 = __mul24(cb0, Sb4);
+                int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
                 bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
 #pragma unroll 2
                 for (int t = t0; t < t1; ++t) {
@@ -869,13 +857,13 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - bb0) * r.inv_kb) : 1.0f;
                     const bool afirst = fa < fb;
                     const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
-                    const int A1 = __mul24(ca1, Sa4), B1 = __mul24(cb1, Sb4);
+                    const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
                     const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
                     const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
-                    if (ia0 && ib0 && w1 > 0.0f) atomicAdd((float*)(ab + (A0 + B0 + row4)), w1 * sv);
+                    if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv, S);
                     const bool i2 = afirst ? (ia1 && ib0) : (ia0 && ib1);
-                    if (i2 && w2 > 0.0f) atomicAdd((float*)(ab + ((afirst ? A1 + B0 : A0 + B1) + row4)), w2 * sv);
-                    if (ia1 && ib1 && w3 > 0.0f) atomicAdd((float*)(ab + (A1 + B1 + row4)), w3 * sv);
+                    if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv, S);
+                    if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv, S);
                     a0 = a1;
                     bb0 = bb1;
                     fla0 = fla1;
@@ -886,7 +874,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     B0 = B1;
                     ia0 = ia1;
                     ib0 = ib1;
-                    row4 += St4;
+                    row += St;
                 }
                 continue;
             }
@@ -903,7 +891,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         if (wk[q] > 0.0f && (unsigned)(ca[q] - la) < ea && (unsigned)(cb[q] - lb) < eb)
-                            atomicAdd(af + (ca[q] * Sa + cb[q] * Sb + row), wk[q] * sv);
+                            lds_deposit<FIX>(ab, ca[q] * Sa + cb[q] * Sb + row, wk[q] * sv, S);
                     }
                 }
                 a0 = a1;
@@ -915,44 +903,62 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         }
     }
     __syncthreads();
+    const float inv_s = FIX ? (S > 0.0f ? 1.0f / S : __builtin_nanf("")) : 1.0f;   // S is a power of two
     for (int i = tid; i < TX * TY * TZ; i += 64 * NW) {
         const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
         if (lx < ext[0] && ly < ext[1] && lz < ext[2]) {
             const size_t o = (size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz));
-            image[o] = accumulate ? image[o] + acc.v[lz][ly][lx] : acc.v[lz][ly][lx];
+            float v;
+            if (FIX)
+                v = (float)((double)(long long)acc.v[lz][ly][lx] * (double)inv_s);
+            else
+                v = (float)acc.v[lz][ly][lx];
+            image[o] = accumulate ? image[o] + v : v;
         }
     }
 }
 
-template <bool REF, int TX, int TY, int TZ, int NW, int CS, bool TABLE>
+template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __global__ __launch_bounds__(64 * NW, 4) void k_siddon_adj_tile(Launch P, const float* __restrict__ pv,
-                                                             float* __restrict__ image, const RayRec* __restrict__ table,
-                                                             int v0, int v1, int accumulate)
+                                                                float* __restrict__ image,
+                                                                const unsigned* __restrict__ maxbits, float sbound)
 {
-    __shared__ TileAcc<TX, TY, TZ> acc;
-    adj_tile_block<REF, TX, TY, TZ, NW, CS, TABLE>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, table, v0,
-                                                   v1, accumulate != 0);
+    __shared__ TileAcc<TX, TY, TZ, acc_t<FIX>> acc;
+    const float S = FIX ? fixed_scale(*maxbits, sbound) : 1.0f;
+    adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, S, false);
 }
 
-// Launch of one tile-adjoint variant over views [v0, v1).
+// Launch of one tile-adjoint variant.
 template <int TX, int TY, int TZ, int NW, int CS>
-void launch_tile(bool ref, const Launch& L, const float* pv, float* image, const RayRec* table, int v0, int v1,
-                 int acc, hipStream_t s)
+void launch_tile(bool ref, bool fix, const Launch& L, const float* pv, float* image, const unsigned* maxbits,
+                 float sbound, hipStream_t s)
 {
     const dim3 g((L.V.n[0] + TX - 1) / TX, (L.V.n[1] + TY - 1) / TY, (L.V.n[2] + TZ - 1) / TZ);
     const dim3 b(64 * NW);
-    if (ref && table)
-        hipLaunchKernelGGL((k_siddon_adj_tile<true, TX, TY, TZ, NW, CS, true>), g, b, 0, s, L, pv, image, table, v0,
-                           v1, acc);
+    if (ref && fix)
+        hipLaunchKernelGGL((k_siddon_adj_tile<true, true, TX, TY, TZ, NW, CS>), g, b, 0, s, L, pv, image, maxbits,
+                           sbound);
     else if (ref)
-        hipLaunchKernelGGL((k_siddon_adj_tile<true, TX, TY, TZ, NW, CS, false>), g, b, 0, s, L, pv, image, table, v0,
-                           v1, acc);
-    else if (table)
-        hipLaunchKernelGGL((k_siddon_adj_tile<false, TX, TY, TZ, NW, CS, true>), g, b, 0, s, L, pv, image, table, v0,
-                           v1, acc);
+        hipLaunchKernelGGL((k_siddon_adj_tile<true, false, TX, TY, TZ, NW, CS>), g, b, 0, s, L, pv, image, maxbits,
+                           sbound);
+    else if (fix)
+        hipLaunchKernelGGL((k_siddon_adj_tile<false, true, TX, TY, TZ, NW, CS>), g, b, 0, s, L, pv, image, maxbits,
+                           sbound);
     else
-        hipLaunchKernelGGL((k_siddon_adj_tile<false, TX, TY, TZ, NW, CS, false>), g, b, 0, s, L, pv, image, table, v0,
-                           v1, acc);
+        hipLaunchKernelGGL((k_siddon_adj_tile<false, false, TX, TY, TZ, NW, CS>), g, b, 0, s, L, pv, image, maxbits,
+                           sbound);
+}
+
+// max |x| over n values, as float bits, into *out (which the caller zeroes);
+// NaN compares above Inf in this order
+__global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ x, size_t n, unsigned* __restrict__ out)
+{
+    unsigned m = 0u;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        m = max(m, __float_as_uint(x[i]) & 0x7fffffffu);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 __global__ void k_view_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
@@ -1084,8 +1090,9 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
     if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
     if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT | NEFCT_F_DETERMINISTIC |
-                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_RECOMPUTE | NEFCT_F_TILE_MASK | NEFCT_F_LANE_STRIDE2))
+                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_FLOAT_ATOMIC | NEFCT_F_TILE_MASK | NEFCT_F_LANE_STRIDE2))
         return NEFCT_ERR_UNSUPPORTED;
+    if ((flags & NEFCT_F_TILE_MASK) > NEFCT_F_TILE_64x16x16) return NEFCT_ERR_UNSUPPORTED;
     if ((flags & NEFCT_F_DETERMINISTIC) && (flags & NEFCT_F_ADJ_WINDOW)) return NEFCT_ERR_UNSUPPORTED;
     if (flags & NEFCT_F_REF_SLAB) {
         // the reference y-walk treats y slabs as x-voxel thick (siddon.cu.cc:188-201)
@@ -1125,6 +1132,7 @@ Launch make_launch(const int grid[3], const float center[3], const float size[3]
     L.trig = (const double*)(ws + p.trig_off);
     L.nv = nv;
     L.flags = flags;
+    L.sbound = scale_bound(L.V, L.G, flags);
     return L;
 }
 
@@ -1202,51 +1210,41 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
                            (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP,
                            image);
     } else {
-        // Volume blocks own their voxels.  Unless told to recompute, the
-        // per-ray set-up is written once per chunk of views into the
-        // workspace (which the padded copies of the forward occupy) and the
-        // blocks read it; a chunk's blocks add to the previous chunks' sums.
-        const bool det = (flags & NEFCT_F_DETERMINISTIC) != 0;
-        const size_t per_view = sizeof(RayRec) * (size_t)na * (size_t)nb;
-        const size_t cap = p.total - p.xf_off;
-        int chunk = (int)std::min<size_t>((size_t)nv, cap / per_view);
-        if (flags & NEFCT_F_ADJ_RECOMPUTE) chunk = 0;
-        RayRec* table = (RayRec*)(ws + p.xf_off);
+        // Volume blocks own their voxels.  Fixed-point accumulation first
+        // needs max |projection| (into a word of the otherwise unused
+        // padded-copy region of the workspace).
+        const bool fix = !(flags & NEFCT_F_ADJ_FLOAT_ATOMIC);
+        unsigned* maxbits = (unsigned*)(ws + p.xf_off);
+        if (fix) {
+            const size_t n = (size_t)nv * (size_t)na * (size_t)nb;
+            launch_zero((float*)maxbits, 1, s);
+            const size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
+            hipLaunchKernelGGL(k_absmax, dim3((unsigned)blocks), dim3(256), 0, s, pv, n, maxbits);
+        }
+        const float sb = L.sbound;
         const unsigned shape = flags & NEFCT_F_TILE_MASK;
         const bool cs2 = (flags & NEFCT_F_LANE_STRIDE2) != 0;
-        auto tile = [&](const RayRec* tb, int v0, int v1, int acc) {
-            if (det) {
-                // one wavefront per 16x16x8 block: fixed LDS summation order
-                launch_tile<16, 16, 8, 1, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else if (shape == NEFCT_F_TILE_32x32x12) {
-                if (cs2) launch_tile<32, 32, 12, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<32, 32, 12, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else if (shape == NEFCT_F_TILE_64x16x8) {
-                if (cs2) launch_tile<64, 16, 8, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<64, 16, 8, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else if (shape == NEFCT_F_TILE_16x16x32) {
-                if (cs2) launch_tile<16, 16, 32, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<16, 16, 32, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else if (shape == NEFCT_F_TILE_64x64x8) {
-                if (cs2) launch_tile<64, 64, 8, 16, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<64, 64, 8, 16, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else if (shape == NEFCT_F_TILE_64x32x16) {
-                if (cs2) launch_tile<64, 32, 16, 16, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<64, 32, 16, 16, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            } else {
-                if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, L, pv, image, tb, v0, v1, acc, s);
-                else launch_tile<32, 32, 8, 4, 1>(ref, L, pv, image, tb, v0, v1, acc, s);
-            }
-        };
-        if (chunk <= 0) {
-            tile(nullptr, 0, nv, 0);
+        if (!fix && (flags & NEFCT_F_DETERMINISTIC)) {
+            // float accumulation in a fixed order: one wavefront per 16x16x8 block
+            launch_tile<16, 16, 8, 1, 1>(ref, false, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_32x16x8) {
+            if (cs2) launch_tile<32, 16, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<32, 16, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_64x32x8) {
+            if (cs2) launch_tile<64, 32, 8, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<64, 32, 8, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_32x32x16) {
+            if (cs2) launch_tile<32, 32, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<32, 32, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_16x16x32) {
+            if (cs2) launch_tile<16, 16, 32, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<16, 16, 32, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_64x16x16) {
+            if (cs2) launch_tile<64, 16, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<64, 16, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         } else {
-            for (int v0 = 0; v0 < nv; v0 += chunk) {
-                const int v1 = std::min(nv, v0 + chunk);
-                hipLaunchKernelGGL(k_ray_table, dim3((na + 255) / 256, nb, v1 - v0), dim3(256), 0, s, L, v0, table,
-                                   1);
-                tile(table, v0, v1, v0 > 0 ? 1 : 0);
-            }
+            if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<32, 32, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         }
     }
     NEFCT_CHECK_LAUNCH();
@@ -1289,6 +1287,7 @@ __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned fla
     L.trig = (const double*)(ws + p.trig_off);
     L.nv = pr.nv;
     L.flags = flags;
+    L.sbound = scale_bound(L.V, L.G, flags);
     return L;
 }
 
@@ -1309,8 +1308,11 @@ __host__ __device__ inline size_t batch_launch_bytes(int nprob) { return align25
 // Per-problem launch parameters live in the workspace: a kernel that picks its
 // problem at run time reads them from global memory (uniform scalar loads)
 // instead of building a private copy, which would be indexed dynamically and
-// so be placed in scratch memory.
+// so be placed in scratch memory.  After them, one max |projection| word per
+// problem (fixed-point adjoint scale).
 __device__ __forceinline__ const Launch* batch_launches(char* ws_base) { return (const Launch*)(ws_base + Q_HEADER); }
+__host__ __device__ inline size_t batch_max_off(int nprob) { return Q_HEADER + batch_launch_bytes(nprob); }
+__host__ __device__ inline size_t batch_head_bytes(int nprob) { return batch_max_off(nprob) + align256(4 * (size_t)nprob); }
 
 // trig tables and launch parameters of every problem + zeroed task counters
 __global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob, char* ws_base, unsigned flags)
@@ -1320,7 +1322,10 @@ __global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob,
     if (p == 0 && i < 8) ((unsigned long long*)ws_base)[i] = 0ull;
     if (p >= nprob) return;
     const nefct_problem& pr = probs[p];
-    if (i == 0) ((Launch*)(ws_base + Q_HEADER))[p] = launch_of(pr, ws_base, flags);
+    if (i == 0) {
+        ((Launch*)(ws_base + Q_HEADER))[p] = launch_of(pr, ws_base, flags);
+        ((unsigned*)(ws_base + batch_max_off(nprob)))[p] = 0u;
+    }
     if (i >= pr.nv) return;
     const WsPlan pl = plan_ws(pr.grid, pr.nv);
     double* trig = (double*)(ws_base + pr.ws_offset + pl.trig_off);
@@ -1353,11 +1358,28 @@ __global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __re
     }
 }
 
+// max |sinogram| of every problem (blockIdx.y = problem)
+__global__ __launch_bounds__(256) void k_batch_absmax(const nefct_problem* __restrict__ probs, int nprob,
+                                                      char* ws_base)
+{
+    const int p = blockIdx.y;
+    if (p >= nprob) return;
+    const nefct_problem& pr = probs[p];
+    const size_t n = (size_t)pr.nv * (size_t)pr.na * (size_t)pr.nb;
+    const float* x = pr.sinogram;
+    unsigned m = 0u;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        m = max(m, __float_as_uint(x[i]) & 0x7fffffffu);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned*)(ws_base + batch_max_off(nprob)) + p, m);
+}
+
 template <bool REF>
-__global__ __launch_bounds__(256, 3) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
+__global__ __launch_bounds__(256, 2) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                    long long total, char* ws_base, unsigned flags)
 {
-    __shared__ TileAcc<BT, BT, BTZ> acc;
+    __shared__ TileAcc<BT, BT, BTZ, acc_t<true>> acc;
     __shared__ long long s_task;
     unsigned long long* counter = (unsigned long long*)ws_base + 1;
     for (;;) {
@@ -1373,7 +1395,9 @@ __global__ __launch_bounds__(256, 3) void k_batch_adj(const nefct_problem* __res
         const int by = (local / pr.blocks[0]) % pr.blocks[1];
         const int bz = local / (pr.blocks[0] * pr.blocks[1]);
         const Launch& L = batch_launches(ws_base)[p];
-        adj_tile_block<REF, BT, BT, BTZ, 4>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc);
+        const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
+        adj_tile_block<REF, true, BT, BT, BTZ, 4, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
+                                                     S, false);
     }
 }
 
@@ -1481,7 +1505,7 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
     if (!probs || !workspace_bytes || !total_tiles || !total_blocks) return NEFCT_ERR_NULL;
     if (nprob <= 0 || nprob > 65535) return NEFCT_ERR_SHAPE;
     long long tiles = 0, blocks = 0;
-    size_t off = Q_HEADER + batch_launch_bytes(nprob);
+    size_t off = batch_head_bytes(nprob);
     for (int i = 0; i < nprob; ++i) {
         nefct_problem& pr = probs[i];
         int st = validate(pr.grid, pr.center, pr.size, pr.angles, pr.nv, pr.SID, pr.SAD, pr.na, pr.da, pr.nb, pr.db,
@@ -1551,7 +1575,12 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
     batch_prep(host, dev, nprob, ws, flags, s);
-    const int g = persistent_grid(3);   // 3 workgroups of 4 waves per CU fit (168 VGPRs)
+    size_t maxn = 1;
+    for (int i = 0; i < nprob; ++i)
+        maxn = std::max(maxn, (size_t)host[i].nv * (size_t)host[i].na * (size_t)host[i].nb);
+    const unsigned bx = (unsigned)std::min<size_t>(1024, (maxn + 255) / 256);
+    hipLaunchKernelGGL(k_batch_absmax, dim3(bx, nprob), dim3(256), 0, s, dev, nprob, ws);
+    const int g = persistent_grid(2);   // two 67.6 KB fixed-point blocks of 4 waves per CU
     if (flags & NEFCT_F_REF_SLAB)
         hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
     else

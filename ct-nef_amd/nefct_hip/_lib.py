@@ -19,14 +19,14 @@ F_SHORT_SKIP = 4
 F_TIE_GT = 8
 F_DETERMINISTIC = 16
 F_ADJ_WINDOW = 32
-F_ADJ_RECOMPUTE = 64
+F_ADJ_FLOAT_ATOMIC = 0x1000
 F_TILE_MASK = 0x700
 F_TILE_32x32x8 = 0x000
-F_TILE_32x32x12 = 0x100
-F_TILE_64x16x8 = 0x200
-F_TILE_16x16x32 = 0x300
-F_TILE_64x64x8 = 0x400
-F_TILE_64x32x16 = 0x500
+F_TILE_32x16x8 = 0x100
+F_TILE_64x32x8 = 0x200
+F_TILE_32x32x16 = 0x300
+F_TILE_16x16x32 = 0x400
+F_TILE_64x16x16 = 0x500
 F_LANE_STRIDE2 = 0x800
 REF_FORWARD = F_INV_L2 | F_REF_SLAB
 REF_ADJOINT = F_REF_SLAB | F_SHORT_SKIP | F_TIE_GT

@@ -2,9 +2,9 @@
 
 The exact adjoint has several implementations with identical weights that
 differ in how work maps onto the chip (include/nefct_hip.h, adjoint flags):
-volume blocks of 32x32x8, 32x32x12, 64x16x8, 16x16x32, 64x64x8 or 64x32x16 voxels, neighbouring
-lanes one or two detector columns apart, per-ray parameters read from a
-table or recomputed, or ray tiles with LDS slab windows.  Which is fastest
+volume blocks of 32x32x8, 32x16x8, 64x32x8, 32x32x16, 16x16x32 or 64x16x16
+voxels, neighbouring lanes one or two detector columns apart, fixed-point or
+float LDS accumulation, or ray tiles with LDS slab windows.  Which is fastest
 depends on the volume, detector and view count, so the tuner times the
 candidates on a sample of the views (timing scales with the view count) and
 keeps the winner per (device, geometry, mode) in a JSON cache.
@@ -28,15 +28,13 @@ from .geometry import ConeGeometry, VolumeGeometry
 ADJOINT_CANDIDATES: Dict[str, int] = {
     "tile32x32x8": _lib.F_TILE_32x32x8,
     "tile32x32x8/stride2": _lib.F_TILE_32x32x8 | _lib.F_LANE_STRIDE2,
-    "tile32x32x8/recompute": _lib.F_TILE_32x32x8 | _lib.F_ADJ_RECOMPUTE,
-    "tile32x32x12": _lib.F_TILE_32x32x12,
-    "tile32x32x12/stride2": _lib.F_TILE_32x32x12 | _lib.F_LANE_STRIDE2,
-    "tile64x16x8": _lib.F_TILE_64x16x8,
-    "tile64x16x8/stride2": _lib.F_TILE_64x16x8 | _lib.F_LANE_STRIDE2,
+    "tile32x16x8": _lib.F_TILE_32x16x8,
+    "tile64x32x8": _lib.F_TILE_64x32x8,
+    "tile64x32x8/stride2": _lib.F_TILE_64x32x8 | _lib.F_LANE_STRIDE2,
+    "tile32x32x16": _lib.F_TILE_32x32x16,
     "tile16x16x32": _lib.F_TILE_16x16x32,
-    "tile64x64x8": _lib.F_TILE_64x64x8,
-    "tile64x64x8/stride2": _lib.F_TILE_64x64x8 | _lib.F_LANE_STRIDE2,
-    "tile64x32x16": _lib.F_TILE_64x32x16,
+    "tile64x16x16": _lib.F_TILE_64x16x16,
+    "tile32x32x8/float": _lib.F_TILE_32x32x8 | _lib.F_ADJ_FLOAT_ATOMIC,
     "window": _lib.F_ADJ_WINDOW,
 }
 

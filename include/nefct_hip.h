@@ -51,28 +51,32 @@ typedef enum {
 #define NEFCT_F_REF_SLAB 2u   /* Q2: reference slab walk and attribution (siddon.cu.cc:41-176) */
 #define NEFCT_F_SHORT_SKIP 4u /* Q3: adjoint skips rays with in-plane length < 10 voxels (:472) */
 #define NEFCT_F_TIE_GT 8u     /* Q6: adjoint x-walk only if |xd| > |yd| (:480) */
-/* Adjoint algorithm selection (ignored by the forward):
- *   default                 volume blocks own their voxels in LDS (no global atomics);
- *                           per-ray parameters come from a table that a set-up
- *                           kernel writes into the workspace, a chunk of views at a time
- *   NEFCT_F_DETERMINISTIC   one wavefront per block: bitwise reproducible result
- *   NEFCT_F_ADJ_WINDOW      ray tiles with LDS slab windows flushed by global atomics
- *   NEFCT_F_ADJ_RECOMPUTE   volume blocks recompute each ray's set-up (no table)
- * All variants give the same weights; only the summation order differs. */
+/* Adjoint algorithm selection (ignored by the forward).  All variants apply
+ * the same weights; they differ in work mapping and summation.
+ *   default                   volume blocks own their voxels in LDS (no global
+ *                             atomics); contributions are summed as 64-bit
+ *                             fixed-point integers with a power-of-two scale
+ *                             from max |projection| (rounding ~1e-9 of the
+ *                             largest contribution): bitwise reproducible
+ *   NEFCT_F_ADJ_FLOAT_ATOMIC  the same blocks summing with LDS float atomics
+ *   NEFCT_F_DETERMINISTIC     with FLOAT_ATOMIC: one wavefront per 16x16x8 block
+ *                             (fixed order); without: no change (already so)
+ *   NEFCT_F_ADJ_WINDOW        ray tiles with LDS slab windows flushed by global
+ *                             float atomics
+ * A non-finite projection value makes the fixed-point result NaN. */
 #define NEFCT_F_DETERMINISTIC 16u
 #define NEFCT_F_ADJ_WINDOW 32u
-#define NEFCT_F_ADJ_RECOMPUTE 64u
+#define NEFCT_F_ADJ_FLOAT_ATOMIC 0x1000u
 /* Block shape of the volume-tile adjoint (x by y by z voxels; 4 wavefronts,
- * 16 for the large blocks, whose LDS array is 133 KB) and the spacing of a
- * wavefront's rays; the autotuner picks among these.  The deterministic
- * variant always uses 16x16x8 blocks of one wavefront. */
+ * 16 for the 16K-voxel blocks, whose fixed-point LDS array is 133 KB) and the
+ * spacing of a wavefront's rays; the autotuner picks among these. */
 #define NEFCT_F_TILE_MASK 0x700u
 #define NEFCT_F_TILE_32x32x8 0x000u  /* default */
-#define NEFCT_F_TILE_32x32x12 0x100u
-#define NEFCT_F_TILE_64x16x8 0x200u
-#define NEFCT_F_TILE_16x16x32 0x300u
-#define NEFCT_F_TILE_64x64x8 0x400u
-#define NEFCT_F_TILE_64x32x16 0x500u
+#define NEFCT_F_TILE_32x16x8 0x100u
+#define NEFCT_F_TILE_64x32x8 0x200u
+#define NEFCT_F_TILE_32x32x16 0x300u
+#define NEFCT_F_TILE_16x16x32 0x400u
+#define NEFCT_F_TILE_64x16x16 0x500u
 #define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector column */
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)

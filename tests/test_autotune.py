@@ -21,7 +21,7 @@ def test_views_bucket_and_cache_roundtrip(tmp_path, monkeypatch):
 def test_candidates_are_valid_flag_sets():
     from nefct_hip import _lib
     from nefct_hip.autotune import ADJOINT_CANDIDATES
-    allowed = (_lib.F_TILE_MASK | _lib.F_LANE_STRIDE2 | _lib.F_ADJ_RECOMPUTE | _lib.F_ADJ_WINDOW)
+    allowed = (_lib.F_TILE_MASK | _lib.F_LANE_STRIDE2 | _lib.F_ADJ_FLOAT_ATOMIC | _lib.F_ADJ_WINDOW)
     for name, fl in ADJOINT_CANDIDATES.items():
         assert fl & ~allowed == 0, name
     assert len(set(ADJOINT_CANDIDATES.values())) == len(ADJOINT_CANDIDATES)

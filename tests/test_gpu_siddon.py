@@ -227,23 +227,23 @@ def test_autograd_matches_adjoint():
     assert torch.allclose(x.grad, ref, rtol=1e-5, atol=1e-6 * ref.abs().max().item())
 
 
-@pytest.mark.parametrize("algo", ["tile", "recompute", "deterministic", "deterministic-recompute", "window",
-                                  "tile32x32x12-stride2", "tile64x16x8", "tile16x16x32", "tile64x64x8",
-                                  "tile64x32x16-stride2", "tile64x64x8-recompute"])
+@pytest.mark.parametrize("algo", ["tile", "float", "deterministic", "deterministic-float", "window",
+                                  "tile32x16x8-stride2", "tile64x32x8", "tile32x32x16", "tile16x16x32",
+                                  "tile64x16x16-stride2", "tile64x32x8-float"])
 def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
-    """Every adjoint variant against the FP64 transpose.  At these sizes the
-    per-ray table of the tile variants holds 4-6 views, so the table path runs
-    over several chunks of views that add to one another."""
+    """Every adjoint variant against the FP64 transpose (fixed-point and
+    float LDS accumulation, every block shape, lane spacing 1 and 2, the
+    slab-window kernel)."""
     nh = _nh()
     from nefct_hip import _lib
-    fl = {"tile": 0, "recompute": _lib.F_ADJ_RECOMPUTE, "deterministic": _lib.F_DETERMINISTIC,
-          "deterministic-recompute": _lib.F_DETERMINISTIC | _lib.F_ADJ_RECOMPUTE,
+    fl = {"tile": 0, "float": _lib.F_ADJ_FLOAT_ATOMIC, "deterministic": _lib.F_DETERMINISTIC,
+          "deterministic-float": _lib.F_DETERMINISTIC | _lib.F_ADJ_FLOAT_ATOMIC,
           "window": _lib.F_ADJ_WINDOW,
-          "tile32x32x12-stride2": _lib.F_TILE_32x32x12 | _lib.F_LANE_STRIDE2,
-          "tile64x16x8": _lib.F_TILE_64x16x8, "tile16x16x32": _lib.F_TILE_16x16x32,
-          "tile64x64x8": _lib.F_TILE_64x64x8,
-          "tile64x32x16-stride2": _lib.F_TILE_64x32x16 | _lib.F_LANE_STRIDE2,
-          "tile64x64x8-recompute": _lib.F_TILE_64x64x8 | _lib.F_ADJ_RECOMPUTE}[algo]
+          "tile32x16x8-stride2": _lib.F_TILE_32x16x8 | _lib.F_LANE_STRIDE2,
+          "tile64x32x8": _lib.F_TILE_64x32x8, "tile32x32x16": _lib.F_TILE_32x32x16,
+          "tile16x16x32": _lib.F_TILE_16x16x32,
+          "tile64x16x16-stride2": _lib.F_TILE_64x16x16 | _lib.F_LANE_STRIDE2,
+          "tile64x32x8-float": _lib.F_TILE_64x32x8 | _lib.F_ADJ_FLOAT_ATOMIC}[algo]
     for gi in (0, 1, 2):
         name, vol, det = _geoms()[gi]
         _, ang, img, ov, od = _case(oracle, vol, det, 16, seed=gi + 21)
@@ -266,3 +266,30 @@ def test_deterministic_adjoint_is_bitwise_reproducible():
     a = nh.backproject(y, vol, det, ang, flags=_lib.F_DETERMINISTIC)
     b = nh.backproject(y, vol, det, ang, flags=_lib.F_DETERMINISTIC)
     assert torch.equal(a, b)
+
+
+def test_fixed_point_adjoint_range_and_special_values(oracle):
+    """The default adjoint sums fixed-point integers scaled from max |y|:
+    bitwise reproducible, exact under power-of-two scaling of y, within 1e-5
+    of the FP64 transpose for y spanning 10 decades, zero for y = 0, and NaN
+    everywhere for a non-finite y."""
+    nh = _nh()
+    name, vol, det = _geoms()[1]
+    _, ang, img, ov, od = _case(oracle, vol, det, 12, seed=41)
+    rng = np.random.default_rng(42)
+    y = rng.uniform(-1, 1, size=(12, det.nb, det.na)).astype(np.float32)
+    yt = torch.tensor(y, device=DEV)
+    a = nh.backproject(yt, vol, det, torch.tensor(ang))
+    b = nh.backproject(yt, vol, det, torch.tensor(ang))
+    assert torch.equal(a, b)
+    big = nh.backproject(yt * 2.0 ** 60, vol, det, torch.tensor(ang))
+    small = nh.backproject(yt * 2.0 ** -60, vol, det, torch.tensor(ang))
+    assert torch.equal(big, a * 2.0 ** 60) and torch.equal(small, a * 2.0 ** -60)
+    y2 = y * (10.0 ** rng.uniform(-10, 0, size=y.shape)).astype(np.float32)
+    got = nh.backproject(torch.tensor(y2, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_bp(ov, od, ang.astype(np.float64), y2.astype(np.float64))
+    assert _relerr(got, ref) < 1e-5
+    assert nh.backproject(torch.zeros_like(yt), vol, det, torch.tensor(ang)).abs().max().item() == 0.0
+    yn = yt.clone()
+    yn[3, 5, 7] = float("nan")
+    assert torch.isnan(nh.backproject(yn, vol, det, torch.tensor(ang))).all()
