@@ -1230,9 +1230,9 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
         } else if (shape == NEFCT_F_TILE_32x16x8) {
             if (cs2) launch_tile<32, 16, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<32, 16, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
-        } else if (shape == NEFCT_F_TILE_64x32x8) {
-            if (cs2) launch_tile<64, 32, 8, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
-            else launch_tile<64, 32, 8, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_32x32x8) {
+            if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<32, 32, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         } else if (shape == NEFCT_F_TILE_32x32x16) {
             if (cs2) launch_tile<32, 32, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<32, 32, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
@@ -1243,8 +1243,8 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
             if (cs2) launch_tile<64, 16, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<64, 16, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         } else {
-            if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
-            else launch_tile<32, 32, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+            if (cs2) launch_tile<64, 32, 8, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<64, 32, 8, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         }
     }
     NEFCT_CHECK_LAUNCH();
@@ -1254,7 +1254,7 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
 // ---------------------------------------------------------------------------
 // Batched problems (persistent kernels fed by an atomic task queue)
 // ---------------------------------------------------------------------------
-constexpr int BT = 32, BTZ = 8;          // adjoint block (same as the plain default)
+constexpr int BTX = 64, BTY = 32, BTZ = 8, BNW = 8;    // adjoint block: the plain default shape, 8 waves
 constexpr int Q_HEADER = 256;            // workspace head: task counters, then one Launch per problem
 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
@@ -1376,10 +1376,10 @@ __global__ __launch_bounds__(256) void k_batch_absmax(const nefct_problem* __res
 }
 
 template <bool REF>
-__global__ __launch_bounds__(256, 2) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
+__global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                    long long total, char* ws_base, unsigned flags)
 {
-    __shared__ TileAcc<BT, BT, BTZ, acc_t<true>> acc;
+    __shared__ TileAcc<BTX, BTY, BTZ, acc_t<true>> acc;
     __shared__ long long s_task;
     unsigned long long* counter = (unsigned long long*)ws_base + 1;
     for (;;) {
@@ -1396,8 +1396,8 @@ __global__ __launch_bounds__(256, 2) void k_batch_adj(const nefct_problem* __res
         const int bz = local / (pr.blocks[0] * pr.blocks[1]);
         const Launch& L = batch_launches(ws_base)[p];
         const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
-        adj_tile_block<REF, true, BT, BT, BTZ, 4, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
-                                                     S, false);
+        adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz,
+                                                         acc, S, false);
     }
 }
 
@@ -1516,8 +1516,8 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
         pr.tiles_b = (pr.nb + BY - 1) / BY;
         pr.tile_begin = tiles;
         tiles += (long long)pr.tiles_a * pr.tiles_b * pr.nv;
-        pr.blocks[0] = (pr.grid[0] + BT - 1) / BT;
-        pr.blocks[1] = (pr.grid[1] + BT - 1) / BT;
+        pr.blocks[0] = (pr.grid[0] + BTX - 1) / BTX;
+        pr.blocks[1] = (pr.grid[1] + BTY - 1) / BTY;
         pr.blocks[2] = (pr.grid[2] + BTZ - 1) / BTZ;
         pr.block_begin = blocks;
         blocks += (long long)pr.blocks[0] * pr.blocks[1] * pr.blocks[2];
@@ -1580,11 +1580,12 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
         maxn = std::max(maxn, (size_t)host[i].nv * (size_t)host[i].na * (size_t)host[i].nb);
     const unsigned bx = (unsigned)std::min<size_t>(1024, (maxn + 255) / 256);
     hipLaunchKernelGGL(k_batch_absmax, dim3(bx, nprob), dim3(256), 0, s, dev, nprob, ws);
-    const int g = persistent_grid(2);   // two 67.6 KB fixed-point blocks of 4 waves per CU
+    const int g = persistent_grid(1);   // one 133 KB fixed-point block per CU
     if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
+        hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
     else
-        hipLaunchKernelGGL((k_batch_adj<false>), dim3(g), dim3(256), 0, s, dev, nprob, total_blocks, ws, flags);
+        hipLaunchKernelGGL((k_batch_adj<false>), dim3(g), dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws,
+                           flags);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }
