@@ -1254,7 +1254,7 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
 // ---------------------------------------------------------------------------
 // Batched problems (persistent kernels fed by an atomic task queue)
 // ---------------------------------------------------------------------------
-constexpr int BTX = 64, BTY = 32, BTZ = 8, BNW = 8;    // adjoint block: the plain default shape, 8 waves
+constexpr int BTX = 64, BTY = 32, BTZ = 8, BNW = 16;   // adjoint block: the plain default
 constexpr int Q_HEADER = 256;            // workspace head: task counters, then one Launch per problem
 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
@@ -1375,30 +1375,31 @@ __global__ __launch_bounds__(256) void k_batch_absmax(const nefct_problem* __res
     if ((threadIdx.x & 63) == 0) atomicMax((unsigned*)(ws_base + batch_max_off(nprob)) + p, m);
 }
 
+// One launch over the voxel blocks of every problem: workgroup g takes global
+// block g and finds its problem by binary search over block_begin.  (A
+// persistent loop over an atomic queue, as the forward uses, holds enough
+// extra state across iterations to make this kernel spill at 16 wavefronts;
+// the blocks are equal in size within a problem, so the dispatcher balances
+// them as well.)
 template <bool REF>
 __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
-                                                   long long total, char* ws_base, unsigned flags)
+                                                        long long total, char* ws_base, unsigned flags)
 {
     __shared__ TileAcc<BTX, BTY, BTZ, acc_t<true>> acc;
-    __shared__ long long s_task;
-    unsigned long long* counter = (unsigned long long*)ws_base + 1;
-    for (;;) {
-        if (threadIdx.x == 0) s_task = (long long)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const long long t = s_task;
-        __syncthreads();
-        if (t >= total) break;
-        const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, true));
-        const nefct_problem& pr = probs[p];
-        const int local = (int)(t - pr.block_begin);
-        const int bx = local % pr.blocks[0];
-        const int by = (local / pr.blocks[0]) % pr.blocks[1];
-        const int bz = local / (pr.blocks[0] * pr.blocks[1]);
-        const Launch& L = batch_launches(ws_base)[p];
-        const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
-        adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz,
-                                                         acc, S, false);
-    }
+    const long long t = (long long)blockIdx.x + 65536ll * (long long)blockIdx.y;
+    if (t >= total) return;
+    const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, true));
+    const nefct_problem& pr = probs[p];
+    // block coordinates made scalar: in vector registers they would carry the
+    // block bounds and their derived values into VGPRs
+    const int local = __builtin_amdgcn_readfirstlane((int)(t - pr.block_begin));
+    const int bx = __builtin_amdgcn_readfirstlane(local % pr.blocks[0]);
+    const int by = __builtin_amdgcn_readfirstlane((local / pr.blocks[0]) % pr.blocks[1]);
+    const int bz = __builtin_amdgcn_readfirstlane(local / (pr.blocks[0] * pr.blocks[1]));
+    const Launch& L = batch_launches(ws_base)[p];
+    const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
+    adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
+                                                     S, false);
 }
 
 // Workgroups of a persistent launch: per_cu per compute unit of the current
@@ -1580,12 +1581,13 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
         maxn = std::max(maxn, (size_t)host[i].nv * (size_t)host[i].na * (size_t)host[i].nb);
     const unsigned bx = (unsigned)std::min<size_t>(1024, (maxn + 255) / 256);
     hipLaunchKernelGGL(k_batch_absmax, dim3(bx, nprob), dim3(256), 0, s, dev, nprob, ws);
-    const int g = persistent_grid(1);   // one 133 KB fixed-point block per CU
+    if (total_blocks <= 0) return NEFCT_OK;
+    if (total_blocks > 65536ll * 65535ll) return NEFCT_ERR_SHAPE;
+    const dim3 g((unsigned)std::min<long long>(total_blocks, 65536), (unsigned)((total_blocks + 65535) / 65536));
     if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_batch_adj<true>), dim3(g), dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+        hipLaunchKernelGGL((k_batch_adj<true>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
     else
-        hipLaunchKernelGGL((k_batch_adj<false>), dim3(g), dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws,
-                           flags);
+        hipLaunchKernelGGL((k_batch_adj<false>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }
