@@ -762,22 +762,33 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         int ia0, ia1, ib0, ib1;
         shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
         if (ia1 < ia0 || ib1 < ib0) continue;
-        // wavefront w takes detector rows ib0 + w, ib0 + w + NW, ...; its lanes
-        // take columns CS apart (CS passes cover a 64 CS column chunk), so a
-        // wavefront's rays are nearly parallel; with CS = 2 neighbouring lanes
-        // are about one voxel apart rather than half a voxel, and their LDS
-        // additions rarely meet on one address
+        // The shadow's W x H detector cells are numbered row by row and dealt
+        // out densely: wavefront w takes cells w*64 + lane + j*64*NW, so no
+        // lane idles for rounding the shadow up to whole 64-column chunks,
+        // and a wavefront's lanes are neighbouring columns of one or two
+        // rows (nearly parallel rays).  CS > 1 interleaves columns instead:
+        // lane l of a 64*CS-cell run takes column CS*l + phase.
         const int wv = tid >> 6, ln = tid & 63;
-        const int cols = CS * ((ia1 - ia0 + 64 * CS) / (64 * CS));
-        const int nk = cols * ((ib1 - ib0 + NW) / NW);
-        // candidate k -> detector cell; the next candidate's projection value
-        // is loaded before the current ray is walked, so the load's latency
-        // overlaps the walk
+        const unsigned W = (unsigned)(ia1 - ia0 + 1), H = (unsigned)(ib1 - ib0 + 1);
+        const unsigned ncell = W * H;
+        const unsigned mW = (W > 1u) ? (unsigned)((0x100000000ull + W - 1) / W) : 0u;   // ceil(2^32 / W)
+        const unsigned groups = (ncell + 64u * CS - 1) / (64u * CS) * CS;   // 64-cell groups to deal out
+        const int nk = (int)((groups + NW - 1) / NW);
         auto cell_of = [&](int k, int& ia, int& ib) {
-            const int kc = k % cols;
-            ib = ib0 + wv + NW * (k / cols);
-            ia = ia0 + (kc / CS) * (64 * CS) + CS * ln + (kc % CS);
-            return ib <= ib1 && ia <= ia1;
+            unsigned e = (unsigned)(wv * 64) + (unsigned)k * (64u * NW);
+            if (CS > 1) {
+                const unsigned run = e / (64u * CS) * (64u * CS);   // e is a multiple of 64
+                e = run + (unsigned)(CS * ln) + (e - run) / 64u;
+            } else {
+                e += (unsigned)ln;
+            }
+            // e / W by multiplication (the quotient may come out one too large)
+            unsigned q = (W > 1u) ? __umulhi(e, mW) : e;
+            unsigned rr = e - q * W;
+            if ((int)rr < 0) { q -= 1; rr += W; }
+            ib = ib0 + (int)q;
+            ia = ia0 + (int)rr;
+            return e < ncell;
         };
         const size_t view_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
         int ia_n = 0, ib_n = 0;
