@@ -652,7 +652,7 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
 // so the result is the transpose of the forward up to summation order.
 //
 // Accumulation (FIX = true, the default): each contribution w * pv * scale
-// is rounded to a signed fixed-point integer with a power-of-two scale S and
+// is converted to a signed fixed-point integer with a power-of-two scale S and
 // added to a 64-bit integer in LDS (ds_add_u64).  S is the largest power of
 // two with |contribution| * S <= 2^30 for every contribution of the call
 // (from max |pv| over the sinogram and a geometric bound on the per-slab ray
@@ -684,11 +684,15 @@ __device__ __forceinline__ float fixed_scale(unsigned maxbits, float sbound)
     return ldexpf(1.0f, e - 1);                 // largest power of two <= 2^30 / big
 }
 
+// Adds contribution v to the LDS word at byte_off.  Fixed point: v already
+// carries the scale S (the ray's value is pre-multiplied by S once), and the
+// conversion truncates toward zero: the per-contribution error is below one
+// unit of 2^-30 of the largest contribution, independent of order.
 template <bool FIX>
-__device__ __forceinline__ void lds_deposit(char* base, int byte_off, float v, float S)
+__device__ __forceinline__ void lds_deposit(char* base, int byte_off, float v)
 {
     if (FIX) {
-        const long long q = (long long)(int)rintf(v * S);
+        const long long q = (long long)(int)v;
         atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), (unsigned long long)q);
     } else {
         atomicAdd(reinterpret_cast<float*>(base + byte_off), v);
@@ -833,7 +837,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
             }
             if (t0 >= t1) continue;
-            const float sv = pval * r.scale;
+            const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
             const Walk w = walk_of(m, P.V, P.Ly);
             // LDS byte offset of centred transverse cell (ca, cb) at slab t:
             // (ca Sa + cb Sb + t St + base) * ES, with the strides of the walk
@@ -871,10 +875,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
                     const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
                     const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
-                    if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv, S);
+                    if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv);
                     const bool i2 = afirst ? (ia1 && ib0) : (ia0 && ib1);
-                    if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv, S);
-                    if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv, S);
+                    if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv);
+                    if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv);
                     a0 = a1;
                     bb0 = bb1;
                     fla0 = fla1;
@@ -902,7 +906,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         if (wk[q] > 0.0f && (unsigned)(ca[q] - la) < ea && (unsigned)(cb[q] - lb) < eb)
-                            lds_deposit<FIX>(ab, ca[q] * Sa + cb[q] * Sb + row, wk[q] * sv, S);
+                            lds_deposit<FIX>(ab, ca[q] * Sa + cb[q] * Sb + row, wk[q] * sv);
                     }
                 }
                 a0 = a1;
@@ -1244,9 +1248,9 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
         } else if (shape == NEFCT_F_TILE_32x32x8) {
             if (cs2) launch_tile<32, 32, 8, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<32, 32, 8, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
-        } else if (shape == NEFCT_F_TILE_32x32x16) {
-            if (cs2) launch_tile<32, 32, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
-            else launch_tile<32, 32, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+        } else if (shape == NEFCT_F_TILE_64x32x8) {
+            if (cs2) launch_tile<64, 32, 8, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<64, 32, 8, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         } else if (shape == NEFCT_F_TILE_16x16x32) {
             if (cs2) launch_tile<16, 16, 32, 4, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<16, 16, 32, 4, 1>(ref, fix, L, pv, image, maxbits, sb, s);
@@ -1254,8 +1258,8 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
             if (cs2) launch_tile<64, 16, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
             else launch_tile<64, 16, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         } else {
-            if (cs2) launch_tile<64, 32, 8, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
-            else launch_tile<64, 32, 8, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
+            if (cs2) launch_tile<32, 32, 16, 16, 2>(ref, fix, L, pv, image, maxbits, sb, s);
+            else launch_tile<32, 32, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         }
     }
     NEFCT_CHECK_LAUNCH();
@@ -1265,7 +1269,7 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
 // ---------------------------------------------------------------------------
 // Batched problems (persistent kernels fed by an atomic task queue)
 // ---------------------------------------------------------------------------
-constexpr int BTX = 64, BTY = 32, BTZ = 8, BNW = 16;   // adjoint block: the plain default
+constexpr int BTX = 32, BTY = 32, BTZ = 16, BNW = 16;  // adjoint block: the plain default
 constexpr int Q_HEADER = 256;            // workspace head: task counters, then one Launch per problem
 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)

@@ -21,10 +21,10 @@ F_DETERMINISTIC = 16
 F_ADJ_WINDOW = 32
 F_ADJ_FLOAT_ATOMIC = 0x1000
 F_TILE_MASK = 0x700
-F_TILE_64x32x8 = 0x000
+F_TILE_32x32x16 = 0x000
 F_TILE_32x16x8 = 0x100
 F_TILE_32x32x8 = 0x200
-F_TILE_32x32x16 = 0x300
+F_TILE_64x32x8 = 0x300
 F_TILE_16x16x32 = 0x400
 F_TILE_64x16x16 = 0x500
 F_LANE_STRIDE2 = 0x800

@@ -26,14 +26,14 @@ from . import _lib
 from .geometry import ConeGeometry, VolumeGeometry
 
 ADJOINT_CANDIDATES: Dict[str, int] = {
-    "tile64x32x8": _lib.F_TILE_64x32x8,
     "tile32x32x16": _lib.F_TILE_32x32x16,
+    "tile64x32x8": _lib.F_TILE_64x32x8,
     "tile64x16x16": _lib.F_TILE_64x16x16,
     "tile32x16x8": _lib.F_TILE_32x16x8,
     "tile32x32x8": _lib.F_TILE_32x32x8,
     "tile16x16x32": _lib.F_TILE_16x16x32,
-    "tile64x32x8/stride2": _lib.F_TILE_64x32x8 | _lib.F_LANE_STRIDE2,
-    "tile64x32x8/float": _lib.F_TILE_64x32x8 | _lib.F_ADJ_FLOAT_ATOMIC,
+    "tile32x32x16/stride2": _lib.F_TILE_32x32x16 | _lib.F_LANE_STRIDE2,
+    "tile32x32x16/float": _lib.F_TILE_32x32x16 | _lib.F_ADJ_FLOAT_ATOMIC,
     "window": _lib.F_ADJ_WINDOW,
 }
 

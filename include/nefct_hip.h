@@ -71,10 +71,10 @@ typedef enum {
  * 16 for the 16K-voxel blocks, whose fixed-point LDS array is 133 KB) and the
  * spacing of a wavefront's rays; the autotuner picks among these. */
 #define NEFCT_F_TILE_MASK 0x700u
-#define NEFCT_F_TILE_64x32x8 0x000u  /* default (16 wavefronts) */
+#define NEFCT_F_TILE_32x32x16 0x000u  /* default (16 wavefronts) */
 #define NEFCT_F_TILE_32x16x8 0x100u
 #define NEFCT_F_TILE_32x32x8 0x200u
-#define NEFCT_F_TILE_32x32x16 0x300u
+#define NEFCT_F_TILE_64x32x8 0x300u
 #define NEFCT_F_TILE_16x16x32 0x400u
 #define NEFCT_F_TILE_64x16x16 0x500u
 #define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector column */
