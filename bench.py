@@ -239,6 +239,16 @@ def main():
         cpu = cpu_baseline(wl, img.cpu(), args.cpu_stride)
         log(f"[bench] CPU baseline {cpu['value']:.4f} Grays/s on {cpu['cores']} threads")
 
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE (scaled by a known-size copy) and committed
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "siddon_pair_c2_traffic.json")
+    if not args.views and os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f)["kernels"].get(dominant)
+        if tk:
+            traffic = tk["traffic_bytes"]
+
     rec = {
         "metric": "Siddon fwd+adjoint Grays/s at 512x512x256, 720 views",
         "value": round(value, 5),
@@ -270,7 +280,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/siddon_pair_c2_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE x copy calibration)"
+            if traffic else None,
             "algorithmic_bytes": b_dom,
             "avg_launch_ms": round(t_dom * 1e3, 3),
         },
