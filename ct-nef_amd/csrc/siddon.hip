@@ -416,12 +416,48 @@ __device__ __forceinline__ void siddon_rays(const Launch& P, int ia, int ib, int
     if (MODE == 2) counts[id] = (int32_t)acc;
 }
 
+// Workgroup order of the forward.  Consecutive workgroup ids go to the 8
+// XCDs in turn (id mod 8), each XCD with its own L2.  The 1-D launch is
+// remapped so that every XCD gets a contiguous range of (ray tile, view)
+// tasks in which the view varies fastest within groups of FWD_VG views: the
+// rays an XCD has in flight then pass through nearly the same voxels (the
+// same detector tile seen from neighbouring angles), and their gathers hit
+// that XCD's L2 instead of re-reading the volume from HBM for every view.
+constexpr int FWD_VG = 16;
+
 template <int MODE, bool REF>
 __global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __restrict__ pv_in,
-                                                     float* __restrict__ pv_out, int32_t* __restrict__ counts)
+                                                     float* __restrict__ pv_out, int32_t* __restrict__ counts,
+                                                     int tiles_a, int tiles_b, int v0, int nvc)
 {
-    siddon_rays<MODE, REF>(P, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z, pv_in,
-                           pv_out, counts);
+    const unsigned per_xcd = gridDim.x >> 3;      // the grid is a multiple of 8
+    const unsigned task = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const unsigned tiles = (unsigned)tiles_a * (unsigned)tiles_b;
+    const unsigned vin = task % FWD_VG, rest = task / FWD_VG;
+    const unsigned tile = rest % tiles, group = rest / tiles;
+    const int iv = (int)(group * FWD_VG + vin);
+    if (iv >= nvc) return;
+    const int ta = (int)(tile % (unsigned)tiles_a), tb = (int)(tile / (unsigned)tiles_a);
+    siddon_rays<MODE, REF>(P, ta * BX + threadIdx.x, tb * BY + threadIdx.y, v0 + iv, pv_in, pv_out, counts);
+}
+
+// Launches k_siddon over all views in chunks whose 1-D grids stay below 2^31
+// work-items.
+template <int MODE, bool REF>
+void launch_siddon(const Launch& L, const float* pv_in, float* pv_out, int32_t* counts, hipStream_t s)
+{
+    const int ta = (L.G.na + BX - 1) / BX, tb = (L.G.nb + BY - 1) / BY;
+    const long long tiles = (long long)ta * tb;
+    long long max_groups = (0x7fffffffll / (BX * BY)) / (tiles * FWD_VG);
+    if (max_groups < 1) max_groups = 1;
+    const int chunk = (int)std::min<long long>(L.nv, max_groups * FWD_VG);
+    for (int v0 = 0; v0 < L.nv; v0 += chunk) {
+        const int nvc = std::min(chunk, L.nv - v0);
+        const long long groups = (nvc + FWD_VG - 1) / FWD_VG;
+        const long long blocks = (tiles * groups * FWD_VG + 7) / 8 * 8;
+        hipLaunchKernelGGL((k_siddon<MODE, REF>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, L, pv_in, pv_out,
+                           counts, ta, tb, v0, nvc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1188,11 +1224,10 @@ int run_forward(const float* image, const int grid[3], const float center[3], co
     Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
     launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
     launch_pad(image, grid, p, ws, s);
-    const dim3 g((na + BX - 1) / BX, (nb + BY - 1) / BY, nv), b(BX, BY);
     if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_siddon<0, true>), g, b, 0, s, L, (const float*)nullptr, out, (int32_t*)nullptr);
+        launch_siddon<0, true>(L, nullptr, out, nullptr, s);
     else
-        hipLaunchKernelGGL((k_siddon<0, false>), g, b, 0, s, L, (const float*)nullptr, out, (int32_t*)nullptr);
+        launch_siddon<0, false>(L, nullptr, out, nullptr, s);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }
@@ -1506,11 +1541,10 @@ int nefct_siddon_count(const int grid[3], const float center[3], const float siz
     char* ws = (char*)workspace;
     Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
     launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
-    const dim3 g((na + BX - 1) / BX, (nb + BY - 1) / BY, nv), b(BX, BY);
     if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_siddon<2, true>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
+        launch_siddon<2, true>(L, nullptr, nullptr, counts, s);
     else
-        hipLaunchKernelGGL((k_siddon<2, false>), g, b, 0, s, L, (const float*)nullptr, (float*)nullptr, counts);
+        launch_siddon<2, false>(L, nullptr, nullptr, counts, s);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }
