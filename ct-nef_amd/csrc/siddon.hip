@@ -1302,10 +1302,10 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
 }
 
 // ---------------------------------------------------------------------------
-// Batched problems (persistent kernels fed by an atomic task queue)
+// Batched problems (one launch per direction over the tasks of every problem)
 // ---------------------------------------------------------------------------
 constexpr int BTX = 32, BTY = 32, BTZ = 16, BNW = 16;  // adjoint block: the plain default
-constexpr int Q_HEADER = 256;            // workspace head: task counters, then one Launch per problem
+constexpr int Q_HEADER = 256;            // workspace head (reserved), then one Launch per problem
 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
 {
@@ -1364,12 +1364,11 @@ __device__ __forceinline__ const Launch* batch_launches(char* ws_base) { return 
 __host__ __device__ inline size_t batch_max_off(int nprob) { return Q_HEADER + batch_launch_bytes(nprob); }
 __host__ __device__ inline size_t batch_head_bytes(int nprob) { return batch_max_off(nprob) + align256(4 * (size_t)nprob); }
 
-// trig tables and launch parameters of every problem + zeroed task counters
+// trig tables and launch parameters of every problem; zeroed max words
 __global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob, char* ws_base, unsigned flags)
 {
     const int p = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && i < 8) ((unsigned long long*)ws_base)[i] = 0ull;
     if (p >= nprob) return;
     const nefct_problem& pr = probs[p];
     if (i == 0) {
@@ -1384,28 +1383,36 @@ __global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob,
     trig[2 * i + 1] = sin(a);
 }
 
+// Forward of every problem in one launch.  Each problem's tasks are numbered
+// as in k_siddon (view fastest within groups of FWD_VG views, then ray tile,
+// then view group; the last group padded to FWD_VG), problems one after the
+// other from tile_begin, so every problem's task count is a multiple of 8.
+// Workgroup ids follow the same order problem by problem, and within a
+// problem they are remapped so that each of the 8 XCDs gets a contiguous
+// eighth of its tasks: all XCDs work on the same problem at a time (one
+// volume in the shared cache, as in one plain call), each on its own views.
+// t0 (a multiple of 8) offsets the chunk this launch covers.
 template <bool REF>
 __global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __restrict__ probs, int nprob,
-                                                       long long total, char* ws_base, unsigned flags)
+                                                       long long t0, long long total, char* ws_base)
 {
-    __shared__ long long s_task;
-    unsigned long long* counter = (unsigned long long*)ws_base;
-    for (;;) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) s_task = (long long)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const long long t = s_task;
-        __syncthreads();
-        if (t >= total) break;
-        const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, false));
-        const nefct_problem& pr = probs[p];
-        const long long local = t - pr.tile_begin;
-        const long long per_view = (long long)pr.tiles_a * pr.tiles_b;
-        const int iv = (int)(local / per_view);
-        const int rem = (int)(local - (long long)iv * per_view);
-        const int tb = rem / pr.tiles_a, ta = rem - tb * pr.tiles_a;
-        const Launch& L = batch_launches(ws_base)[p];
-        siddon_rays<0, REF>(L, ta * BX + threadIdx.x, tb * BY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
-    }
+    const long long id = t0 + blockIdx.x;
+    if (id >= total) return;
+    const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, id, false));
+    const nefct_problem& pr = probs[p];
+    const long long begin = pr.tile_begin;
+    const long long n = (p + 1 < nprob ? probs[p + 1].tile_begin : total) - begin;
+    const long long lid = id - begin;
+    const long long local = (lid & 7) * (n >> 3) + (lid >> 3);
+    const unsigned tiles = (unsigned)pr.tiles_a * (unsigned)pr.tiles_b;
+    const unsigned vin = (unsigned)(local % FWD_VG);
+    const long long rest = local / FWD_VG;
+    const unsigned tile = (unsigned)(rest % tiles);
+    const int iv = (int)((rest / tiles) * FWD_VG + vin);
+    if (iv >= pr.nv) return;
+    const int ta = (int)(tile % (unsigned)pr.tiles_a), tb = (int)(tile / (unsigned)pr.tiles_a);
+    const Launch& L = batch_launches(ws_base)[p];
+    siddon_rays<0, REF>(L, ta * BX + threadIdx.x, tb * BY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
 }
 
 // max |sinogram| of every problem (blockIdx.y = problem)
@@ -1427,10 +1434,9 @@ __global__ __launch_bounds__(256) void k_batch_absmax(const nefct_problem* __res
 
 // One launch over the voxel blocks of every problem: workgroup g takes global
 // block g and finds its problem by binary search over block_begin.  (A
-// persistent loop over an atomic queue, as the forward uses, holds enough
-// extra state across iterations to make this kernel spill at 16 wavefronts;
-// the blocks are equal in size within a problem, so the dispatcher balances
-// them as well.)
+// persistent loop over an atomic queue holds enough extra state across
+// iterations to make this kernel spill at 16 wavefronts; the blocks are equal
+// in size within a problem, so the dispatcher balances them as well.)
 template <bool REF>
 __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                         long long total, char* ws_base, unsigned flags)
@@ -1450,22 +1456,6 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
     const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
     adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
                                                      S, false);
-}
-
-// Workgroups of a persistent launch: per_cu per compute unit of the current
-// device (queried once per device; safe to call while a graph is captured).
-int persistent_grid(int per_cu)
-{
-    static int cached[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * per_cu;
-    if (cached[dev] == 0) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        cached[dev] = cus;
-    }
-    return cached[dev] * per_cu;
 }
 
 }  // namespace
@@ -1565,7 +1555,7 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
         pr.tiles_a = (pr.na + BX - 1) / BX;
         pr.tiles_b = (pr.nb + BY - 1) / BY;
         pr.tile_begin = tiles;
-        tiles += (long long)pr.tiles_a * pr.tiles_b * pr.nv;
+        tiles += (long long)pr.tiles_a * pr.tiles_b * ((pr.nv + FWD_VG - 1) / FWD_VG * FWD_VG);
         pr.blocks[0] = (pr.grid[0] + BTX - 1) / BTX;
         pr.blocks[1] = (pr.grid[1] + BTY - 1) / BTY;
         pr.blocks[2] = (pr.grid[2] + BTZ - 1) / BTZ;
@@ -1605,11 +1595,17 @@ int nefct_batch_project(const nefct_problem* host, const nefct_problem* dev, int
         const WsPlan p = plan_ws(host[i].grid, host[i].nv);
         launch_pad(host[i].volume, host[i].grid, p, ws + host[i].ws_offset, s);
     }
-    const int g = persistent_grid(8);
-    if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_batch_fwd<true>), dim3(g), dim3(BX, BY), 0, s, dev, nprob, total_tiles, ws, flags);
-    else
-        hipLaunchKernelGGL((k_batch_fwd<false>), dim3(g), dim3(BX, BY), 0, s, dev, nprob, total_tiles, ws, flags);
+    // chunks of at most 2^31 / 256 workgroups (1-D grid of 32-bit work-items)
+    const long long chunk = (0x7fffffffll / (BX * BY)) / 8 * 8;
+    for (long long t0 = 0; t0 < total_tiles; t0 += chunk) {
+        const long long blocks = std::min(chunk, total_tiles - t0);
+        if (flags & NEFCT_F_REF_SLAB)
+            hipLaunchKernelGGL((k_batch_fwd<true>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, dev, nprob, t0,
+                               total_tiles, ws);
+        else
+            hipLaunchKernelGGL((k_batch_fwd<false>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, dev, nprob, t0,
+                               total_tiles, ws);
+    }
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }

@@ -1,9 +1,10 @@
 """Batched Siddon pair over many problems of uneven size.
 
-One persistent launch per direction covers every problem: the forward hands
-out 64x4-ray tasks (one view) and the adjoint 32x32x16 voxel blocks from an
-atomic queue kept in the workspace, so small problems do not leave the GPU
-idle and there is one launch instead of one per problem (the reference loops
+One launch per direction covers every problem: the forward's workgroups are
+64x4-ray tasks (one view, neighbouring views dealt to the same XCD) and the
+adjoint's 32x32x16 voxel blocks, numbered across all problems, so small
+problems do not leave the GPU idle and there is one launch instead of one per
+problem (the reference loops
 in Python over views and phases, nefct/functions/project_t.py:153-159,
 back_project_t.py:150-156).
 

@@ -133,8 +133,8 @@ int nefct_siddon_count(const int grid[3], const float center[3], const float siz
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------- */
-/* Batched Siddon pair: many problems of uneven size in one persistent     */
-/* launch per direction.  No reference counterpart: it replaces the        */
+/* Batched Siddon pair: many problems of uneven size in one launch per     */
+/* direction.            No reference counterpart: it replaces the        */
 /* reference's per-view Python loops (nefct/functions/project_t.py:153-159, */
 /* back_project_t.py:150-156) and per-problem launches.                     */
 /* ---------------------------------------------------------------------- */
@@ -152,8 +152,8 @@ typedef struct {
     int na, nb;
     float da, ai, db, bi;
     /* filled by nefct_batch_plan */
-    long long tile_begin;   /* first forward task (64x4 rays of one view) */
-    long long block_begin;  /* first adjoint task (one 32x32x8 voxel block) */
+    long long tile_begin;   /* first forward task (64x4 rays of one view; views padded to groups of 16) */
+    long long block_begin;  /* first adjoint task (one 32x32x16 voxel block) */
     long long ws_offset;    /* byte offset of the problem's workspace region */
     int tiles_a, tiles_b;
     int blocks[3];
@@ -165,15 +165,15 @@ typedef struct {
 int nefct_batch_plan(nefct_problem* problems, int nprob, unsigned flags, size_t* workspace_bytes,
                      long long* total_tiles, long long* total_blocks);
 
-/* Forward projection of every problem: one trig/counter pass, the padded
- * volume copies, then one persistent kernel whose workgroups take 64x4-ray
- * tasks from an atomic queue in the workspace.  'host_problems' is the
+/* Forward projection of every problem: one trig/launch-record pass, the
+ * padded volume copies, then one launch over the 64x4-ray tasks of all
+ * problems, dealt to the XCDs in contiguous ranges.  'host_problems' is the
  * planned table, 'device_problems' a device copy of it. */
 int nefct_batch_project(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
                         long long total_tiles, unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
 
-/* Exact (or reference-mode) adjoint of every problem: one persistent kernel
- * whose workgroups take 32x32x8 voxel blocks from an atomic queue.  Every
+/* Exact (or reference-mode) adjoint of every problem: one max-|projection|
+ * pass, then one launch with a workgroup per 32x32x16 voxel block.  Every
  * output volume is fully written. */
 int nefct_batch_backproject(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
                             long long total_blocks, unsigned flags, void* workspace, size_t workspace_bytes,
