@@ -735,6 +735,13 @@ __device__ __forceinline__ void lds_deposit(char* base, int byte_off, float v)
     }
 }
 
+// Fixed-point deposit of a nonzero quantum q whose sign word is hi (0 or -1).
+__device__ __forceinline__ void lds_deposit_q(char* base, int byte_off, int q, int hi)
+{
+    const unsigned long long v = ((unsigned long long)(unsigned)hi << 32) | (unsigned)q;
+    atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), v);
+}
+
 __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
                                               const int hi[3], int& ia0, int& ia1, int& ib0, int& ib1)
 {
@@ -896,6 +903,8 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 // once per slab, not once per segment.  Same arithmetic as
                 // slab_segments<false>.
                 int ca0 = (int)fla0, cb0 = (int)flb0;
+                const int qs = FIX ? (int)sv : 0;               // |sv| <= 2^30
+                const int hi = (FIX && sv < 0.0f) ? -1 : 0;
                 int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
                 bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
 #pragma unroll 2
@@ -910,11 +919,24 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
                     const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
                     const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
-                    const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
-                    if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv);
                     const bool i2 = afirst ? (ia1 && ib0) : (ia0 && ib1);
-                    if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv);
-                    if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv);
+                    if constexpr (FIX) {
+                        // the three pieces as differences of the truncated
+                        // prefix sums f1 sv, f2 sv, sv: they add up to the
+                        // slab's whole contribution exactly, and all share
+                        // the ray's sign, so zeros are skipped and the high
+                        // word is the ray's constant
+                        const int q1 = (int)(f1 * sv), q2 = (int)(f2 * sv);
+                        const int d2 = q2 - q1, d3 = qs - q2;
+                        if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, A0 + B0 + row, q1, hi);
+                        if (i2 && d2 != 0) lds_deposit_q(ab, (afirst ? A1 + B0 : A0 + B1) + row, d2, hi);
+                        if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, A1 + B1 + row, d3, hi);
+                    } else {
+                        const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
+                        if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv);
+                        if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv);
+                        if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv);
+                    }
                     a0 = a1;
                     bb0 = bb1;
                     fla0 = fla1;
