@@ -233,20 +233,31 @@ __device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts&
 // their fractions of the slab; the exact rule walks from the slab's lower
 // face, the reference rule (NEFCT_F_REF_SLAB) measures each axis from its
 // low-value end (siddon.cu.cc:52-76, 132-176).
+//
+// Exact rule: the fraction of the slab walked before an axis crosses its
+// next cell boundary, clamp01((boundary - entry) / slope), is >= 1 when the
+// axis stays in its cell, so the clamp alone gives 1 there.  Rounding can
+// leave it just below 1 without a crossing; the slab's length is then split
+// between two pieces of the same cell, whose sum is unchanged.  GUARD tests
+// the index change explicitly instead, for counting voxels (each once).
 struct Seg {
     int a1, b1, a2, b2, a3, b3;
     float w1, w2, w3;
     bool ok;     // reference outer range test (siddon.cu.cc:80,112,132)
 };
 
-template <bool REF>
+template <bool REF, bool GUARD = false>
 __device__ __forceinline__ void slab_segments(float a0, float a1, float fla0, float fla1, float b0, float b1,
                                               float flb0, float flb1, const Ray& r, const Walk& w, Seg& s)
 {
     const int ca0 = (int)fla0, ca1 = (int)fla1, cb0 = (int)flb0, cb1 = (int)flb1;
     if (!REF) {
-        const float fa = (ca1 != ca0) ? clamp01((fla0 + r.pa - a0) * r.inv_ka) : 1.0f;
-        const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - b0) * r.inv_kb) : 1.0f;
+        float fa = clamp01((fla0 + r.pa - a0) * r.inv_ka);
+        float fb = clamp01((flb0 + r.pb - b0) * r.inv_kb);
+        if (GUARD) {
+            fa = (ca1 != ca0) ? fa : 1.0f;
+            fb = (cb1 != cb0) ? fb : 1.0f;
+        }
         const bool afirst = fa < fb;
         const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
         s.a1 = ca0; s.b1 = cb0;
@@ -302,7 +313,7 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
         const float a1 = fmaf(r.ka, tf, r.Ac), b1 = fmaf(r.kb, tf, r.Bc);
         const float fla1 = floorf(a1), flb1 = floorf(b1);
         Seg s;
-        slab_segments<REF>(a0, a1, fla0, fla1, b0, b1, flb0, flb1, r, w, s);
+        slab_segments<REF, MODE == 2>(a0, a1, fla0, fla1, b0, b1, flb0, flb1, r, w, s);
         const bool act = mine && (t >= r.s0) && (t < r.s1) && s.ok;
         const int o1 = (s.a1 + __mul24(s.b1, w.sb) + tb) * 4;
         const int o2 = (s.a2 + __mul24(s.b2, w.sb) + tb) * 4;
@@ -700,8 +711,12 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
 // ---------------------------------------------------------------------------
 template <int TX, int TY, int TZ, typename T>
 struct TileAcc {
-    static constexpr int SX = TX + 1;  // odd row stride: lanes on consecutive y rows hit distinct banks
-    T v[TZ][TY][SX];
+    // odd row and plane strides (in elements): lanes on consecutive y rows or
+    // z planes hit distinct banks
+    static constexpr int SX = TX + 1;
+    static constexpr int SXY = TY * SX + ((TY * SX) % 2 == 0 ? 1 : 0);
+    T v[TZ * SXY];
+    __device__ T& at(int x, int y, int z) { return v[z * SXY + y * SX + x]; }
 };
 
 template <bool FIX>
@@ -794,11 +809,12 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
 {
     using T = acc_t<FIX>;
     constexpr int SX = TileAcc<TX, TY, TZ, T>::SX;
+    constexpr int SXY = TileAcc<TX, TY, TZ, T>::SXY;
     constexpr int ES = (int)sizeof(T);
     const int tid = threadIdx.x;
-    T* af = &acc.v[0][0][0];
+    T* af = &acc.v[0];
     __syncthreads();   // the previous block's write-out has read acc
-    for (int i = tid; i < TZ * TY * SX; i += 64 * NW) af[i] = (T)0;
+    for (int i = tid; i < TZ * SXY; i += 64 * NW) af[i] = (T)0;
     const int nx = P.V.n[0], ny = P.V.n[1], nz = P.V.n[2];
     const int b0[3] = {bx * TX, by * TY, bz * TZ};
     const int ext[3] = {min(TX, nx - b0[0]), min(TY, ny - b0[1]), min(TZ, nz - b0[2])};
@@ -809,16 +825,21 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         int ia0, ia1, ib0, ib1;
         shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
         if (ia1 < ia0 || ib1 < ib0) continue;
-        // The shadow's W x H detector cells are numbered row by row and dealt
-        // out densely: wavefront w takes cells w*64 + lane + j*64*NW, so no
-        // lane idles for rounding the shadow up to whole 64-column chunks,
-        // and a wavefront's lanes are neighbouring columns of one or two
-        // rows (nearly parallel rays).  CS > 1 interleaves columns instead:
-        // lane l of a 64*CS-cell run takes column CS*l + phase.
+        // The shadow's W x H detector cells are numbered column by column
+        // and dealt out densely: wavefront w takes cells w*64 + lane +
+        // j*64*NW, so no lane idles for rounding the shadow up to whole
+        // 64-cell chunks.  A wavefront's lanes are then neighbouring rows of
+        // one or two detector columns: rays of one column share their path in
+        // the plane of the source trajectory (flat and cylindrical detectors
+        // alike), so they cross the block over nearly the same slab range and
+        // finish the walk together (numbering row by row mixes rays through
+        // the block's middle with rays grazing its edges in one wavefront).
+        // CS > 1 interleaves rows instead: lane l of a 64*CS-cell run takes
+        // row CS*l + phase.
         const int wv = tid >> 6, ln = tid & 63;
         const unsigned W = (unsigned)(ia1 - ia0 + 1), H = (unsigned)(ib1 - ib0 + 1);
         const unsigned ncell = W * H;
-        const unsigned mW = (W > 1u) ? (unsigned)((0x100000000ull + W - 1) / W) : 0u;   // ceil(2^32 / W)
+        const unsigned mH = (H > 1u) ? (unsigned)((0x100000000ull + H - 1) / H) : 0u;   // ceil(2^32 / H)
         const unsigned groups = (ncell + 64u * CS - 1) / (64u * CS) * CS;   // 64-cell groups to deal out
         const int nk = (int)((groups + NW - 1) / NW);
         auto cell_of = [&](int k, int& ia, int& ib) {
@@ -829,12 +850,12 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             } else {
                 e += (unsigned)ln;
             }
-            // e / W by multiplication (the quotient may come out one too large)
-            unsigned q = (W > 1u) ? __umulhi(e, mW) : e;
-            unsigned rr = e - q * W;
-            if ((int)rr < 0) { q -= 1; rr += W; }
-            ib = ib0 + (int)q;
-            ia = ia0 + (int)rr;
+            // e / H by multiplication (the quotient may come out one too large)
+            unsigned q = (H > 1u) ? __umulhi(e, mH) : e;
+            unsigned rr = e - q * H;
+            if ((int)rr < 0) { q -= 1; rr += H; }
+            ia = ia0 + (int)q;
+            ib = ib0 + (int)rr;
             return e < ncell;
         };
         const size_t view_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
@@ -886,7 +907,6 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             // (ca Sa + cb Sb + t St + base) * ES, with the strides of the walk
             // axis; [t0, t1) lies inside the block, so only ca and cb are
             // range-checked
-            constexpr int SXY = TY * SX;
             const int Sa = ES * ((m == 0) ? SX : 1);
             const int Sb = ES * ((m == 2) ? SX : SXY);
             const int St = ES * ((m == 0) ? 1 : ((m == 1) ? SX : SXY));
@@ -913,8 +933,8 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
                     const float fla1 = floorf(a1), flb1 = floorf(bb1);
                     const int ca1 = (int)fla1, cb1 = (int)flb1;
-                    const float fa = (ca1 != ca0) ? clamp01((fla0 + r.pa - a0) * r.inv_ka) : 1.0f;
-                    const float fb = (cb1 != cb0) ? clamp01((flb0 + r.pb - bb0) * r.inv_kb) : 1.0f;
+                    const float fa = clamp01((fla0 + r.pa - a0) * r.inv_ka);
+                    const float fb = clamp01((flb0 + r.pb - bb0) * r.inv_kb);
                     const bool afirst = fa < fb;
                     const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
                     const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
@@ -983,9 +1003,9 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             const size_t o = (size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz));
             float v;
             if (FIX)
-                v = (float)((double)(long long)acc.v[lz][ly][lx] * (double)inv_s);
+                v = (float)((double)(long long)acc.at(lx, ly, lz) * (double)inv_s);
             else
-                v = (float)acc.v[lz][ly][lx];
+                v = (float)acc.at(lx, ly, lz);
             image[o] = accumulate ? image[o] + v : v;
         }
     }

@@ -3,7 +3,7 @@
 The exact adjoint has several implementations with identical weights that
 differ in how work maps onto the chip (include/nefct_hip.h, adjoint flags):
 volume blocks of 32x32x8, 32x16x8, 64x32x8, 32x32x16, 16x16x32 or 64x16x16
-voxels, neighbouring lanes one or two detector columns apart, fixed-point or
+voxels, neighbouring lanes one or two detector rows apart, fixed-point or
 float LDS accumulation, or ray tiles with LDS slab windows.  Which is fastest
 depends on the volume, detector and view count, so the tuner times the
 candidates on a sample of the views (timing scales with the view count) and

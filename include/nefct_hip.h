@@ -77,7 +77,7 @@ typedef enum {
 #define NEFCT_F_TILE_64x32x8 0x300u
 #define NEFCT_F_TILE_16x16x32 0x400u
 #define NEFCT_F_TILE_64x16x16 0x500u
-#define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector column */
+#define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector row */
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
 
