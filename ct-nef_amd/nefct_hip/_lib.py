@@ -51,6 +51,12 @@ SIGNATURES = {
     "nefct_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nefct_deform_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nefct_deform_dvf_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "nefct_deform_tex": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "nefct_invert_dvf": [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "nefct_deform_invert_tex": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp],
+    "nefct_shift_tex": [_vp, _vp, _f, _f, _f, _vp, _vp],
+    "nefct_shift_z_tex": [_vp, _vp, _f, _vp, _vp],
+    "nefct_rotate_tex": [_vp, _vp, _f, _f, _vp, _vp],
     "nefct_fdk_preweight": [_vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _f, _i, _f, _vp],
     "nefct_fdk_filter": [_vp, _vp, _i, _i, _vp, _vp],
     "nefct_fdk_workspace_size": [_i],

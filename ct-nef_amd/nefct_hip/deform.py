@@ -113,3 +113,112 @@ class Warp(torch.autograd.Function):
 
 def warp(image, dvf, boundary: int = 1):
     return Warp.apply(image, dvf, boundary)
+
+
+# --------------------------------------------------------------------------
+# Texture-semantics warps (nefct/cc/deform/deform_tex.cc:12-66): clamp-to-edge
+# trilinear sampling with the sample coordinate i + 1/2 at voxel i.
+# --------------------------------------------------------------------------
+def _same_shape(image, *fields):
+    for m in fields:
+        if m.shape != image.shape:
+            raise _lib.NefctError(f"displacement shape {tuple(m.shape)} != image shape {tuple(image.shape)}")
+
+
+@torch.library.custom_op("nefct::deform_tex", mutates_args=())
+def deform_tex_op(image: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor) -> torch.Tensor:
+    _need_device(image, mx, my, mz)
+    _same_shape(image, mx, my, mz)
+    image, mx, my, mz = (t.contiguous() for t in (image, mx, my, mz))
+    out = torch.empty_like(image)
+    st = _lib.load().nefct_deform_tex(_ptr(image), _ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(image)),
+                                      _ptr(out), _vp(_stream_ptr(image.device)))
+    _lib.check(st, "deform_tex")
+    return out
+
+
+@deform_tex_op.register_fake
+def _(image, mx, my, mz):
+    return torch.empty_like(image)
+
+
+@torch.library.custom_op("nefct::invert_dvf", mutates_args=())
+def invert_dvf_op(mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor, iterations: int) -> torch.Tensor:
+    """Returns [3, *shape]: the fixed-point inverse field."""
+    _need_device(mx, my, mz)
+    _same_shape(mx, my, mz)
+    mx, my, mz = (t.contiguous() for t in (mx, my, mz))
+    out = torch.empty((3,) + tuple(mx.shape), dtype=torch.float32, device=mx.device)
+    st = _lib.load().nefct_invert_dvf(_ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(mx)), int(iterations),
+                                      _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _vp(_stream_ptr(mx.device)))
+    _lib.check(st, "invert_dvf")
+    return out
+
+
+@invert_dvf_op.register_fake
+def _(mx, my, mz, iterations):
+    return mx.new_empty((3,) + tuple(mx.shape))
+
+
+@torch.library.custom_op("nefct::shift_tex", mutates_args=())
+def shift_tex_op(image: torch.Tensor, mx: float, my: float, mz: float) -> torch.Tensor:
+    _need_device(image)
+    image = image.contiguous()
+    out = torch.empty_like(image)
+    st = _lib.load().nefct_shift_tex(_ptr(image), _lib.int3(_grid(image)), float(mx), float(my), float(mz),
+                                     _ptr(out), _vp(_stream_ptr(image.device)))
+    _lib.check(st, "shift_tex")
+    return out
+
+
+@shift_tex_op.register_fake
+def _(image, mx, my, mz):
+    return torch.empty_like(image)
+
+
+@torch.library.custom_op("nefct::rotate_tex", mutates_args=())
+def rotate_tex_op(image: torch.Tensor, cos_phi: float, sin_phi: float) -> torch.Tensor:
+    _need_device(image)
+    image = image.contiguous()
+    out = torch.empty_like(image)
+    st = _lib.load().nefct_rotate_tex(_ptr(image), _lib.int3(_grid(image)), float(cos_phi), float(sin_phi),
+                                      _ptr(out), _vp(_stream_ptr(image.device)))
+    _lib.check(st, "rotate_tex")
+    return out
+
+
+@rotate_tex_op.register_fake
+def _(image, cos_phi, sin_phi):
+    return torch.empty_like(image)
+
+
+def deform_tex(image, mx, my, mz):
+    """out[i] = image(i + m(i)), clamp-to-edge trilinear (deform_tex.cu.cc:15-26)."""
+    return deform_tex_op(image, mx, my, mz)
+
+
+def invert_dvf(mx, my, mz, iterations: int = 30):
+    """Fixed-point inverse of a displacement field (deform_tex.cu.cc:78-100);
+    returns [3, *shape]."""
+    return invert_dvf_op(mx, my, mz, int(iterations))
+
+
+def deform_invert_tex(image, mx, my, mz):
+    """Warp by the inverse of m (deform_tex.cu.cc:170-226)."""
+    inv = invert_dvf_op(mx, my, mz, 30)
+    return deform_tex_op(image, inv[0], inv[1], inv[2])
+
+
+def shift_tex(image, mx: float, my: float, mz: float):
+    """out[i] = image(i - (mx, my, mz)) (deform_tex.cu.cc:228-239)."""
+    return shift_tex_op(image, float(mx), float(my), float(mz))
+
+
+def shift_z_tex(image, mz: float):
+    """Shift along z only (deform_tex.cu.cc:291-302)."""
+    return shift_tex_op(image, 0.0, 0.0, float(mz))
+
+
+def rotate_tex(image, cos_phi: float, sin_phi: float):
+    """In-plane rotation about (nx/2, ny/2) (deform_tex.cu.cc:355-370)."""
+    return rotate_tex_op(image, float(cos_phi), float(sin_phi))

@@ -14,6 +14,9 @@ cpp/deform/deform.cc:12-22), same names, argument order and meaning:
   (siddon.cc:263, siddon.cu.cc:527).
 * ``Deform(image, mx, my, mz, grid)`` -> warped image, reference boundary rule,
   zero-filled output (nefct/cc/deform/deform.cc:73-107).
+* ``deform_tex.Deform / DeformInvert / Shift / ShiftZ / Rotate`` -- the ops of
+  the texture warp library (nefct/cc/deform/deform_tex.cc:12-66): clamp-to-edge
+  trilinear sampling, fixed-point inverse displacement field (30 iterations).
 
 ``grid``/``center``/``size`` may be lists or tensors (on any device); they are
 read on the host once.  Inputs must be HIP-device float32 tensors whose memory
@@ -39,7 +42,7 @@ import torch
 
 from . import _lib
 from . import ops
-from .deform import deform_op
+from .deform import deform_invert_tex, deform_op, deform_tex_op, rotate_tex_op, shift_tex_op
 from .geometry import ConeGeometry, VolumeGeometry
 from .siddon import backproject, project
 
@@ -103,6 +106,46 @@ def Deform(image, mx, my, mz, grid):
     shp = image.shape
     v = lambda t: t.reshape(g[2], g[1], g[0])
     return deform_op(v(image), v(mx), v(my), v(mz), 0).reshape(shp)
+
+
+class _DeformTexOps:
+    """The ops of the reference's texture warp library
+    (nefct/cc/deform/deform_tex.cc:12-66), same names and arguments; volumes
+    are reshaped to (nz, ny, nx) from ``grid`` = (nx, ny, nz)."""
+
+    @staticmethod
+    def _v(t, g):
+        return t.reshape(g[2], g[1], g[0])
+
+    def Deform(self, image, mx, my, mz, grid):
+        """Replaces the Deform op of deform_tex.cc:12-22 (deform_tex.cu.cc:29-76)."""
+        g = _host3(grid, int)
+        v = lambda t: self._v(t, g)
+        return deform_tex_op(v(image), v(mx), v(my), v(mz)).reshape(image.shape)
+
+    def DeformInvert(self, image, mx, my, mz, grid):
+        """Replaces DeformInvert (deform_tex.cc:24-34, deform_tex.cu.cc:170-226)."""
+        g = _host3(grid, int)
+        v = lambda t: self._v(t, g)
+        return deform_invert_tex(v(image), v(mx), v(my), v(mz)).reshape(image.shape)
+
+    def Shift(self, image, grid, mx, my, mz):
+        """Replaces Shift (deform_tex.cc:36-46, deform_tex.cu.cc:242-289)."""
+        g = _host3(grid, int)
+        return shift_tex_op(self._v(image, g), float(mx), float(my), float(mz)).reshape(image.shape)
+
+    def ShiftZ(self, image, grid, mz):
+        """Replaces ShiftZ (deform_tex.cc:48-56, deform_tex.cu.cc:306-353)."""
+        g = _host3(grid, int)
+        return shift_tex_op(self._v(image, g), 0.0, 0.0, float(mz)).reshape(image.shape)
+
+    def Rotate(self, image, grid, cos_phi, sin_phi):
+        """Replaces Rotate (deform_tex.cc:58-66, deform_tex.cu.cc:373-420)."""
+        g = _host3(grid, int)
+        return rotate_tex_op(self._v(image, g), float(cos_phi), float(sin_phi)).reshape(image.shape)
+
+
+deform_tex = _DeformTexOps()
 
 
 # --------------------------------------------------------------------------

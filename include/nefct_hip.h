@@ -206,6 +206,51 @@ int nefct_deform_dvf_grad(const float* image, const float* grad_out, const float
                           int boundary, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Texture-semantics warps (nefct/cc/deform/deform_tex.{cc,cu.cc})         */
+/* ---------------------------------------------------------------------- */
+/* Trilinear sampling with the reference's texture rule (deform_tex.cu.cc:60-69:
+ * unnormalised coordinates, linear filter, clamp addressing): along each axis
+ * coordinate u covers texel i on [i, i+1); value (1-a) T[clamp(i0)] +
+ * a T[clamp(i0+1)], i0 = floor(u - 1/2), a = u - 1/2 - i0, clamp to [0, n-1].
+ * Weights are exact FP32 (texture hardware rounds a to 1/256).  Volumes x
+ * fastest, displacements in voxel units; no allocation, no synchronisation. */
+
+/* out[i] = T(i + m(i) + 1/2).  Replaces deform_tex (deform_tex.cu.cc:29-76,
+ * DeformKernel :15-26) behind op Deform (deform_tex.cc:12-22). */
+int nefct_deform_tex(const float* image, const float* mx, const float* my, const float* mz, const int grid[3],
+                     float* deformed_image, void* stream);
+
+/* Fixed-point inverse of a displacement field: per voxel d = 0, then
+ * 'iterations' times x = -mx(i + d), y = -my(i + d), z = -mz(i + d), each
+ * sample using the components already updated.  Replaces invert
+ * (deform_tex.cu.cc:102-168, DeformInvertKernel :78-100; 30 iterations there). */
+int nefct_invert_dvf(const float* mx, const float* my, const float* mz, const int grid[3], int iterations,
+                     float* inv_mx, float* inv_my, float* inv_mz, void* stream);
+
+/* Warp by the inverted field (30 iterations).  workspace >= 12 * voxels bytes
+ * (the inverted field).  Replaces deform_invert_tex (deform_tex.cu.cc:170-226)
+ * behind op DeformInvert (deform_tex.cc:24-34). */
+int nefct_deform_invert_tex(const float* image, const float* mx, const float* my, const float* mz, const int grid[3],
+                            float* deformed_image, void* workspace, size_t workspace_bytes, void* stream);
+
+/* out[i] = T(i - (mx, my, mz) + 1/2).  Replaces shift_tex (deform_tex.cu.cc:
+ * 242-289, ShiftKernel :228-239) behind op Shift (deform_tex.cc:36-46). */
+int nefct_shift_tex(const float* image, const int grid[3], float mx, float my, float mz, float* deformed_image,
+                    void* stream);
+
+/* Shift along z only.  Replaces shift_z_tex (deform_tex.cu.cc:306-353,
+ * ShiftZKernel :291-302) behind op ShiftZ (deform_tex.cc:48-56). */
+int nefct_shift_z_tex(const float* image, const int grid[3], float mz, float* deformed_image, void* stream);
+
+/* In-plane rotation about the integer centre (nx/2, ny/2): sample at
+ * ((ix+1/2-nx/2) cos - (iy+1/2-ny/2) sin + nx/2, (ix+1/2-nx/2) sin +
+ * (iy+1/2-ny/2) cos + ny/2, iz + 1/2).  Replaces rotate_tex
+ * (deform_tex.cu.cc:373-420, RotateKernel :355-370) behind op Rotate
+ * (deform_tex.cc:58-66). */
+int nefct_rotate_tex(const float* image, const int grid[3], float cos_phi, float sin_phi, float* deformed_image,
+                     void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* FDK (flat detector)                                                     */
 /* ---------------------------------------------------------------------- */
 

@@ -26,6 +26,10 @@ Contents
       ``nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:154``.
     - anisotropic TV ``nefct/functions/total_variation.py:19-35`` and its
       adjoint ``:125-149``.
+    - texture-rule warps ``nefct/cc/deform/deform_tex.cu.cc``: ``tex_sample``
+      (clamp-to-edge trilinear), ``deform_tex``, ``invert_dvf`` (fixed-point
+      inverse field), ``deform_invert_tex``, ``shift_tex``, ``shift_z_tex``,
+      ``rotate_tex``.
 
 Parity status of each function is stated in its docstring; the checks that pin
 it live in ``tests/test_oracle_*.py``.
@@ -495,3 +499,104 @@ def fdk_fp_bp(cyl, grid, angles, offsets, SID, SAD, na, da, ai, nb, db, bi, pv):
                         int(na), float(da), float(ai), int(nb), float(db), float(bi), _ptr(pv, ctypes.c_double),
                         _ptr(out, ctypes.c_double))
     return out
+
+
+# --------------------------------------------------------------------------
+# Texture-sampled warps (nefct/cc/deform/deform_tex.cu.cc)
+# --------------------------------------------------------------------------
+def tex_sample(vol_zyx, u, v, w):
+    """Trilinear sample with the semantics of the reference's 3-D texture
+    (deform_tex.cu.cc:60-69: unnormalised coordinates, linear filtering,
+    clamp addressing): along each axis the sample coordinate u covers texel i
+    on [i, i + 1), so the value is (1 - a) T[clamp(i0)] + a T[clamp(i0 + 1)]
+    with i0 = floor(u - 1/2), a = u - 1/2 - i0 and clamp to [0, n - 1].
+    Weights are exact (FP64); the hardware's 1/256 quantisation of a is not
+    modelled.  u, v, w are arrays of one shape (x, y, z coordinates)."""
+    vol = np.asarray(vol_zyx, np.float64)
+    nz, ny, nx = vol.shape
+    out = np.zeros(np.shape(u), np.float64)
+    axes = []
+    for c, n in ((u, nx), (v, ny), (w, nz)):
+        xb = np.asarray(c, np.float64) - 0.5
+        i0 = np.floor(xb)
+        a = xb - i0
+        i0 = i0.astype(np.int64)
+        axes.append((np.clip(i0, 0, n - 1), np.clip(i0 + 1, 0, n - 1), a))
+    (x0, x1, ax), (y0, y1, ay), (z0, z1, az) = axes
+    for zi, wz in ((z0, 1 - az), (z1, az)):
+        for yi, wy in ((y0, 1 - ay), (y1, ay)):
+            for xi, wx in ((x0, 1 - ax), (x1, ax)):
+                out += vol[zi, yi, xi] * wx * wy * wz
+    return out
+
+
+def _tex_grid(shape_zyx):
+    nz, ny, nx = shape_zyx
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return ix, iy, iz
+
+
+def _f32(*xs):
+    """Sum in float32 left to right, as the reference forms its coordinates."""
+    acc = np.float32(0) + np.asarray(xs[0], np.float32)
+    for x in xs[1:]:
+        acc = (acc + np.asarray(x, np.float32)).astype(np.float32)
+    return acc.astype(np.float64)
+
+
+def deform_tex(img_zyx, mx, my, mz):
+    """DeformKernel, deform_tex.cu.cc:15-26: out[i] = T(i + m(i) + 1/2)."""
+    ix, iy, iz = _tex_grid(np.shape(img_zyx))
+    return tex_sample(img_zyx, _f32(ix, mx, 0.5), _f32(iy, my, 0.5), _f32(iz, mz, 0.5))
+
+
+def invert_dvf(mx, my, mz, iters=30):
+    """DeformInvertKernel, deform_tex.cu.cc:78-100: per voxel i the fixed-point
+    iteration x = -mx(i + d), y = -my(i + d), z = -mz(i + d) with d = (x, y, z)
+    updated component by component (each sample uses the components already
+    updated in this iteration), starting from d = 0, 30 iterations.  FP64."""
+    mx, my, mz = (np.asarray(a, np.float64) for a in (mx, my, mz))
+    ix, iy, iz = _tex_grid(mx.shape)
+    x = np.zeros(mx.shape)
+    y = np.zeros(mx.shape)
+    z = np.zeros(mx.shape)
+    for _ in range(iters):
+        x = -tex_sample(mx, x + ix + 0.5, y + iy + 0.5, z + iz + 0.5)
+        y = -tex_sample(my, x + ix + 0.5, y + iy + 0.5, z + iz + 0.5)
+        z = -tex_sample(mz, x + ix + 0.5, y + iy + 0.5, z + iz + 0.5)
+    return x, y, z
+
+
+def deform_invert_tex(img_zyx, mx, my, mz, iters=30):
+    """deform_invert_tex, deform_tex.cu.cc:170-226: warp by the inverted field."""
+    x, y, z = invert_dvf(mx, my, mz, iters)
+    return deform_tex(img_zyx, x, y, z)
+
+
+def shift_tex(img_zyx, sx, sy, sz):
+    """ShiftKernel, deform_tex.cu.cc:228-239: out[i] = T(i - s + 1/2)."""
+    ix, iy, iz = _tex_grid(np.shape(img_zyx))
+    return tex_sample(img_zyx, _f32(ix, -np.float32(sx), 0.5), _f32(iy, -np.float32(sy), 0.5),
+                      _f32(iz, -np.float32(sz), 0.5))
+
+
+def shift_z_tex(img_zyx, sz):
+    """ShiftZKernel, deform_tex.cu.cc:291-302: shift along z only."""
+    ix, iy, iz = _tex_grid(np.shape(img_zyx))
+    return tex_sample(img_zyx, ix + 0.5, iy + 0.5, _f32(iz, -np.float32(sz), 0.5))
+
+
+def rotate_tex(img_zyx, cos_phi, sin_phi):
+    """RotateKernel, deform_tex.cu.cc:355-370: in-plane rotation about the
+    integer centre (nx/2, ny/2) (integer division), sampled at
+    (x', y', iz + 1/2) with x' = (ix + 1/2 - nx/2) cos - (iy + 1/2 - ny/2) sin + nx/2,
+    y' = (ix + 1/2 - nx/2) sin + (iy + 1/2 - ny/2) cos + ny/2 (float32, as the
+    reference evaluates them)."""
+    nz, ny, nx = np.shape(img_zyx)
+    ix, iy, iz = _tex_grid((nz, ny, nx))
+    c, s = np.float32(cos_phi), np.float32(sin_phi)
+    ax = (ix.astype(np.float32) + np.float32(0.5) - np.float32(nx // 2)).astype(np.float32)
+    ay = (iy.astype(np.float32) + np.float32(0.5) - np.float32(ny // 2)).astype(np.float32)
+    xp = ((ax * c).astype(np.float32) - (ay * s).astype(np.float32)).astype(np.float32) + np.float32(nx // 2)
+    yp = ((ax * s).astype(np.float32) + (ay * c).astype(np.float32)).astype(np.float32) + np.float32(ny // 2)
+    return tex_sample(img_zyx, xp.astype(np.float64), yp.astype(np.float64), iz + 0.5)
