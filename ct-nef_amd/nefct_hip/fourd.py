@@ -1,8 +1,8 @@
 """Warp-then-project over many problems (4-D CT phases, uneven batches).
 
 Each problem p projects ``warp(base_p, dvf_p)`` with its own views.  Forward:
-one warp launch per problem, then one persistent batched projection for all
-problems.  Backward: one persistent batched adjoint, then the warp's
+one warp launch per problem, then one batched projection launch for all
+problems.  Backward: one batched adjoint launch, then the warp's
 transpose (gradient to the base volume) and its displacement gradient per
 problem.  The warped volumes live only in the batch's static buffers during
 the forward; backward recomputes what it needs from the base volumes and
