@@ -877,28 +877,41 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             int t0 = max(r.s0, b0[m]), t1 = min(r.s1, b1[m]);
             if (t0 >= t1) continue;
             // FP32 coordinates of the ray at the block's slab faces decide the
-            // transverse overlap (one cell of margin), as the forward would see them
+            // transverse overlap, as the walk will compute them.  The box is
+            // the block widened by MARGIN cells: a slab deposits only into
+            // cells between floor(a(t)) and floor(a(t+1)), so it matters only
+            // if the walk's position there reaches the block's range; the
+            // walk's FP32 positions are within 2^-24 * 512 cells of the exact
+            // line and the slab estimates below within 600 * 2^-23 * |1/k|
+            // slabs, both far inside MARGIN (and its MARGIN / |k| slabs) plus
+            // the one slab of slack on each side.
+            constexpr float MARGIN = 1.0f / 64.0f;
             const float fa0 = fmaf(r.ka, (float)(t0 - r.tc), r.Ac), fa1 = fmaf(r.ka, (float)(t1 - r.tc), r.Ac);
             const float fb0 = fmaf(r.kb, (float)(t0 - r.tc), r.Bc), fb1 = fmaf(r.kb, (float)(t1 - r.tc), r.Bc);
             const int ha = P.V.n[a] >> 1, hb = P.V.n[b] >> 1;
-            const float alo = (float)(b0[a] - ha - 1), ahi = (float)(b1[a] - ha + 1);
-            const float blo = (float)(b0[b] - hb - 1), bhi = (float)(b1[b] - hb + 1);
+            const float alo = (float)(b0[a] - ha) - MARGIN, ahi = (float)(b1[a] - ha) + MARGIN;
+            const float blo = (float)(b0[b] - hb) - MARGIN, bhi = (float)(b1[b] - hb) + MARGIN;
             if (fmaxf(fa0, fa1) < alo || fminf(fa0, fa1) > ahi || fmaxf(fb0, fb1) < blo || fminf(fb0, fb1) > bhi)
                 continue;
-            // tighten [t0, t1) to the slabs where the ray is within the margin box
-            // (the clamps keep the result while bounding the conversion:
-            // t0 and t1 already lie in [b0[m], b1[m]])
+            // tighten [t0, t1) to the slabs where the ray is within the margin
+            // box, with one slab of slack on each side (the clamps keep the
+            // result while bounding the conversion: t0 and t1 already lie in
+            // [b0[m], b1[m]]; a NaN estimate, 0 * inf, leaves the range as is)
             if (r.ka != 0.0f) {
                 float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                if (ta == ta && tb == tb) {
+                    t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                    t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                }
             }
             if (r.kb != 0.0f) {
                 float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                if (ta == ta && tb == tb) {
+                    t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                    t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                }
             }
             if (t0 >= t1) continue;
             const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
