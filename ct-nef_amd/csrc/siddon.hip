@@ -32,8 +32,10 @@
 namespace {
 
 constexpr int PAD = 2;          // zero border of the transverse axes
-constexpr int BX = 64;          // detector columns per wavefront
-constexpr int BY = 4;           // detector rows per workgroup (one per wave)
+constexpr int BX = 64;          // window adjoint: detector columns per wavefront
+constexpr int BY = 4;           // window adjoint: detector rows per workgroup (one per wave)
+constexpr int FX = 16;          // forward: detector columns per workgroup (x fastest: a wavefront is 16 x 4 rays)
+constexpr int FY = 16;          // forward: detector rows per workgroup
 
 struct VolDev {
     int n[3];
@@ -405,7 +407,7 @@ __device__ float project_all_dominances(const Launch& P, const Ray& r, bool insi
     return acc;
 }
 
-// One workgroup of 64 x 4 rays (columns ia, rows ib) of view iv.
+// One workgroup of FX x FY rays (columns ia, rows ib) of view iv.
 template <int MODE, bool REF>
 __device__ __forceinline__ void siddon_rays(const Launch& P, int ia, int ib, int iv, const float* __restrict__ pv_in,
                                             float* __restrict__ pv_out, int32_t* __restrict__ counts)
@@ -430,26 +432,34 @@ __device__ __forceinline__ void siddon_rays(const Launch& P, int ia, int ib, int
 // Workgroup order of the forward.  Consecutive workgroup ids go to the 8
 // XCDs in turn (id mod 8), each XCD with its own L2.  The 1-D launch is
 // remapped so that every XCD gets a contiguous range of (ray tile, view)
-// tasks in which the view varies fastest within groups of FWD_VG views: the
-// rays an XCD has in flight then pass through nearly the same voxels (the
+// tasks in which the view varies fastest within groups of 2^vg_log2 views:
+// the rays an XCD has in flight then pass through nearly the same voxels (the
 // same detector tile seen from neighbouring angles), and their gathers hit
 // that XCD's L2 instead of re-reading the volume from HBM for every view.
-constexpr int FWD_VG = 16;
+// The group spans about 16 degrees of a full scan (fwd_group_log2).
+constexpr int FWD_VG = 16;   // view group of the batched forward
+
+__host__ inline int fwd_group_log2(int nv)
+{
+    int g = 2;                                  // 4 views at least ...
+    while (g < 5 && (nv >> (g + 1)) >= 22) ++g; // ... up to 32 while 2^g views span <= 360/22 degrees
+    return g;
+}
 
 template <int MODE, bool REF>
-__global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __restrict__ pv_in,
+__global__ __launch_bounds__(FX * FY) void k_siddon(Launch P, const float* __restrict__ pv_in,
                                                      float* __restrict__ pv_out, int32_t* __restrict__ counts,
-                                                     int tiles_a, int tiles_b, int v0, int nvc)
+                                                     int tiles_a, int tiles_b, int v0, int nvc, int vg_log2)
 {
     const unsigned per_xcd = gridDim.x >> 3;      // the grid is a multiple of 8
     const unsigned task = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
     const unsigned tiles = (unsigned)tiles_a * (unsigned)tiles_b;
-    const unsigned vin = task % FWD_VG, rest = task / FWD_VG;
+    const unsigned vin = task & ((1u << vg_log2) - 1u), rest = task >> vg_log2;
     const unsigned tile = rest % tiles, group = rest / tiles;
-    const int iv = (int)(group * FWD_VG + vin);
+    const int iv = (int)((group << vg_log2) + vin);
     if (iv >= nvc) return;
     const int ta = (int)(tile % (unsigned)tiles_a), tb = (int)(tile / (unsigned)tiles_a);
-    siddon_rays<MODE, REF>(P, ta * BX + threadIdx.x, tb * BY + threadIdx.y, v0 + iv, pv_in, pv_out, counts);
+    siddon_rays<MODE, REF>(P, ta * FX + threadIdx.x, tb * FY + threadIdx.y, v0 + iv, pv_in, pv_out, counts);
 }
 
 // Launches k_siddon over all views in chunks whose 1-D grids stay below 2^31
@@ -457,17 +467,18 @@ __global__ __launch_bounds__(BX * BY) void k_siddon(Launch P, const float* __res
 template <int MODE, bool REF>
 void launch_siddon(const Launch& L, const float* pv_in, float* pv_out, int32_t* counts, hipStream_t s)
 {
-    const int ta = (L.G.na + BX - 1) / BX, tb = (L.G.nb + BY - 1) / BY;
+    const int ta = (L.G.na + FX - 1) / FX, tb = (L.G.nb + FY - 1) / FY;
     const long long tiles = (long long)ta * tb;
-    long long max_groups = (0x7fffffffll / (BX * BY)) / (tiles * FWD_VG);
+    const int vgl = fwd_group_log2(L.nv), vg = 1 << vgl;
+    long long max_groups = (0x7fffffffll / (FX * FY)) / (tiles * vg);
     if (max_groups < 1) max_groups = 1;
-    const int chunk = (int)std::min<long long>(L.nv, max_groups * FWD_VG);
+    const int chunk = (int)std::min<long long>(L.nv, max_groups * vg);
     for (int v0 = 0; v0 < L.nv; v0 += chunk) {
         const int nvc = std::min(chunk, L.nv - v0);
-        const long long groups = (nvc + FWD_VG - 1) / FWD_VG;
-        const long long blocks = (tiles * groups * FWD_VG + 7) / 8 * 8;
-        hipLaunchKernelGGL((k_siddon<MODE, REF>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, L, pv_in, pv_out,
-                           counts, ta, tb, v0, nvc);
+        const long long groups = (nvc + vg - 1) / vg;
+        const long long blocks = (tiles * groups * vg + 7) / 8 * 8;
+        hipLaunchKernelGGL((k_siddon<MODE, REF>), dim3((unsigned)blocks), dim3(FX, FY), 0, s, L, pv_in, pv_out,
+                           counts, ta, tb, v0, nvc, vgl);
     }
 }
 
@@ -1448,7 +1459,7 @@ __global__ void k_batch_prep(const nefct_problem* __restrict__ probs, int nprob,
 // volume in the shared cache, as in one plain call), each on its own views.
 // t0 (a multiple of 8) offsets the chunk this launch covers.
 template <bool REF>
-__global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __restrict__ probs, int nprob,
+__global__ __launch_bounds__(FX * FY) void k_batch_fwd(const nefct_problem* __restrict__ probs, int nprob,
                                                        long long t0, long long total, char* ws_base)
 {
     const long long id = t0 + blockIdx.x;
@@ -1467,7 +1478,7 @@ __global__ __launch_bounds__(BX * BY) void k_batch_fwd(const nefct_problem* __re
     if (iv >= pr.nv) return;
     const int ta = (int)(tile % (unsigned)pr.tiles_a), tb = (int)(tile / (unsigned)pr.tiles_a);
     const Launch& L = batch_launches(ws_base)[p];
-    siddon_rays<0, REF>(L, ta * BX + threadIdx.x, tb * BY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
+    siddon_rays<0, REF>(L, ta * FX + threadIdx.x, tb * FY + threadIdx.y, iv, nullptr, pr.sinogram, nullptr);
 }
 
 // max |sinogram| of every problem (blockIdx.y = problem)
@@ -1607,8 +1618,8 @@ int nefct_batch_plan(nefct_problem* probs, int nprob, unsigned flags, size_t* wo
                           pr.ai, pr.bi, flags);
         if (st) return st;
         if (!pr.volume || !pr.sinogram) return NEFCT_ERR_NULL;
-        pr.tiles_a = (pr.na + BX - 1) / BX;
-        pr.tiles_b = (pr.nb + BY - 1) / BY;
+        pr.tiles_a = (pr.na + FX - 1) / FX;
+        pr.tiles_b = (pr.nb + FY - 1) / FY;
         pr.tile_begin = tiles;
         tiles += (long long)pr.tiles_a * pr.tiles_b * ((pr.nv + FWD_VG - 1) / FWD_VG * FWD_VG);
         pr.blocks[0] = (pr.grid[0] + BTX - 1) / BTX;
@@ -1651,14 +1662,14 @@ int nefct_batch_project(const nefct_problem* host, const nefct_problem* dev, int
         launch_pad(host[i].volume, host[i].grid, p, ws + host[i].ws_offset, s);
     }
     // chunks of at most 2^31 / 256 workgroups (1-D grid of 32-bit work-items)
-    const long long chunk = (0x7fffffffll / (BX * BY)) / 8 * 8;
+    const long long chunk = (0x7fffffffll / (FX * FY)) / 8 * 8;
     for (long long t0 = 0; t0 < total_tiles; t0 += chunk) {
         const long long blocks = std::min(chunk, total_tiles - t0);
         if (flags & NEFCT_F_REF_SLAB)
-            hipLaunchKernelGGL((k_batch_fwd<true>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, dev, nprob, t0,
+            hipLaunchKernelGGL((k_batch_fwd<true>), dim3((unsigned)blocks), dim3(FX, FY), 0, s, dev, nprob, t0,
                                total_tiles, ws);
         else
-            hipLaunchKernelGGL((k_batch_fwd<false>), dim3((unsigned)blocks), dim3(BX, BY), 0, s, dev, nprob, t0,
+            hipLaunchKernelGGL((k_batch_fwd<false>), dim3((unsigned)blocks), dim3(FX, FY), 0, s, dev, nprob, t0,
                                total_tiles, ws);
     }
     NEFCT_CHECK_LAUNCH();
