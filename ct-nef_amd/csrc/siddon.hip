@@ -79,6 +79,26 @@ struct Ray {
 
 __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
+// Cell of a walk coordinate and the position inside it: floor(x) as an int
+// and x - floor(x) (v_fract_f32).  The coordinates are bounded (|x| < 2^20).
+// (v_cvt_flr_i32_f32 would give the int in one instruction, but as inline
+// assembly it is treated as convergent, which stops the walk loops from being
+// unrolled; the unroll is worth more.)
+__device__ __forceinline__ int cell_floor(float x) { return (int)floorf(x); }
+__device__ __forceinline__ float cell_frac(float x) { return __builtin_amdgcn_fractf(x); }
+
+// Exact rule, one transverse axis: fraction of the slab walked before the
+// coordinate leaves its cell, (pa - frac) / k = fma(frac, -1/k, pa/k) with
+// pa = 1 for k > 0 (next boundary above) and 0 for k < 0, clamped to [0, 1].
+// It is >= 1 when the coordinate stays in its cell, so the clamp alone gives
+// 1 there; rounding can leave it just below 1 without a crossing, which
+// splits the slab's length between two pieces of the same cell (sum
+// unchanged).
+__device__ __forceinline__ float cross_frac(float frac, float inv_k, float pa_inv_k)
+{
+    return clamp01(fmaf(frac, -inv_k, pa_inv_k));
+}
+
 // floor / ceil of v as an int clamped to [lo, hi] before the conversion, so
 // huge or non-finite values (slopes near zero) cannot overflow; NaN gives lo
 __device__ __forceinline__ int floor_clamp(float v, int lo, int hi)
@@ -232,60 +252,58 @@ __device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts&
 }
 
 // Per-slab segment decomposition.  Returns the three (centred) cells and
-// their fractions of the slab; the exact rule walks from the slab's lower
-// face, the reference rule (NEFCT_F_REF_SLAB) measures each axis from its
-// low-value end (siddon.cu.cc:52-76, 132-176).
-//
-// Exact rule: the fraction of the slab walked before an axis crosses its
-// next cell boundary, clamp01((boundary - entry) / slope), is >= 1 when the
-// axis stays in its cell, so the clamp alone gives 1 there.  Rounding can
-// leave it just below 1 without a crossing; the slab's length is then split
-// between two pieces of the same cell, whose sum is unchanged.  GUARD tests
-// the index change explicitly instead, for counting voxels (each once).
+// their fractions of the slab.  The exact rule walks from the slab's lower
+// face (slab_exact: entry cell, the cell after the first crossing, exit
+// cell); the reference rule (NEFCT_F_REF_SLAB, slab_segments) measures each
+// axis from its low-value end (siddon.cu.cc:52-76, 132-176).  GUARD tests the
+// index change explicitly instead of relying on the clamp (cross_frac), for
+// counting voxels (each once).
 struct Seg {
     int a1, b1, a2, b2, a3, b3;
     float w1, w2, w3;
     bool ok;     // reference outer range test (siddon.cu.cc:80,112,132)
 };
 
-template <bool REF, bool GUARD = false>
+template <bool GUARD>
+__device__ __forceinline__ void slab_exact(int ca0, int ca1, float fra0, int cb0, int cb1, float frb0, const Ray& r,
+                                           float pia, float pib, Seg& s)
+{
+    float fa = cross_frac(fra0, r.inv_ka, pia);
+    float fb = cross_frac(frb0, r.inv_kb, pib);
+    if (GUARD) {
+        fa = (ca1 != ca0) ? fa : 1.0f;
+        fb = (cb1 != cb0) ? fb : 1.0f;
+    }
+    const bool afirst = fa < fb;
+    const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
+    s.a1 = ca0; s.b1 = cb0;
+    s.a2 = afirst ? ca1 : ca0;
+    s.b2 = afirst ? cb0 : cb1;
+    s.a3 = ca1; s.b3 = cb1;
+    s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
+    s.ok = true;
+}
+
 __device__ __forceinline__ void slab_segments(float a0, float a1, float fla0, float fla1, float b0, float b1,
-                                              float flb0, float flb1, const Ray& r, const Walk& w, Seg& s)
+                                              float flb0, float flb1, const Walk& w, Seg& s)
 {
     const int ca0 = (int)fla0, ca1 = (int)fla1, cb0 = (int)flb0, cb1 = (int)flb1;
-    if (!REF) {
-        float fa = clamp01((fla0 + r.pa - a0) * r.inv_ka);
-        float fb = clamp01((flb0 + r.pb - b0) * r.inv_kb);
-        if (GUARD) {
-            fa = (ca1 != ca0) ? fa : 1.0f;
-            fb = (cb1 != cb0) ? fb : 1.0f;
-        }
-        const bool afirst = fa < fb;
-        const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
-        s.a1 = ca0; s.b1 = cb0;
-        s.a2 = afirst ? ca1 : ca0;
-        s.b2 = afirst ? cb0 : cb1;
-        s.a3 = ca1; s.b3 = cb1;
-        s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
-        s.ok = true;
-    } else {
-        const float loa = fminf(a0, a1), hia = fmaxf(a0, a1);
-        const float lob = fminf(b0, b1), hib = fmaxf(b0, b1);
-        const int cla = min(ca0, ca1), cha = max(ca0, ca1);
-        const int clb = min(cb0, cb1), chb = max(cb0, cb1);
-        const float ra = (cla != cha) ? ((float)cha - loa) / (hia - loa) : 1.0f;
-        const float rb = (clb != chb) ? ((float)chb - lob) / (hib - lob) : 1.0f;
-        const int tla = cla + w.ha, tlb = clb + w.hb;
-        const bool oka = (cla == cha) ? (tla >= 0 && tla < w.na) : (tla >= -1 && tla < w.na);
-        const bool okb = (clb == chb) ? (tlb >= 0 && tlb < w.nb) : (tlb >= -1 && tlb < w.nb);
-        const float f1 = fminf(ra, rb), f2 = fmaxf(ra, rb);
-        s.a1 = cla; s.b1 = clb;
-        s.a2 = (ra > rb) ? cla : cha;
-        s.b2 = (ra > rb) ? chb : clb;
-        s.a3 = cha; s.b3 = chb;
-        s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
-        s.ok = oka && okb;
-    }
+    const float loa = fminf(a0, a1), hia = fmaxf(a0, a1);
+    const float lob = fminf(b0, b1), hib = fmaxf(b0, b1);
+    const int cla = min(ca0, ca1), cha = max(ca0, ca1);
+    const int clb = min(cb0, cb1), chb = max(cb0, cb1);
+    const float ra = (cla != cha) ? ((float)cha - loa) / (hia - loa) : 1.0f;
+    const float rb = (clb != chb) ? ((float)chb - lob) / (hib - lob) : 1.0f;
+    const int tla = cla + w.ha, tlb = clb + w.hb;
+    const bool oka = (cla == cha) ? (tla >= 0 && tla < w.na) : (tla >= -1 && tla < w.na);
+    const bool okb = (clb == chb) ? (tlb >= 0 && tlb < w.nb) : (tlb >= -1 && tlb < w.nb);
+    const float f1 = fminf(ra, rb), f2 = fmaxf(ra, rb);
+    s.a1 = cla; s.b1 = clb;
+    s.a2 = (ra > rb) ? cla : cha;
+    s.b2 = (ra > rb) ? chb : clb;
+    s.a3 = cha; s.b3 = chb;
+    s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
+    s.ok = oka && okb;
 }
 
 __device__ __forceinline__ bool cell_in(const Walk& w, int ca, int cb)
@@ -306,16 +324,31 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
     float acc = 0.0f;
     float tf = (float)(tmin - r.tc);
     float a0 = fmaf(r.ka, tf, r.Ac), b0 = fmaf(r.kb, tf, r.Bc);
-    float fla0 = floorf(a0), flb0 = floorf(b0);
     const float sv = val * r.scale;
     int tb = tmin * w.st + w.org;
+    // per slab: entry state (a0/b0 and their cells) carried from the previous slab
+    float fla0 = floorf(a0), flb0 = floorf(b0);               // reference rule
+    int ca0 = cell_floor(a0), cb0 = cell_floor(b0);           // exact rule
+    float fra0 = cell_frac(a0), frb0 = cell_frac(b0);
+    const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
 #pragma unroll 2
     for (int t = tmin; t < tmax; ++t) {
         tf += 1.0f;
         const float a1 = fmaf(r.ka, tf, r.Ac), b1 = fmaf(r.kb, tf, r.Bc);
-        const float fla1 = floorf(a1), flb1 = floorf(b1);
         Seg s;
-        slab_segments<REF, MODE == 2>(a0, a1, fla0, fla1, b0, b1, flb0, flb1, r, w, s);
+        float fla1 = 0.0f, flb1 = 0.0f, fra1 = 0.0f, frb1 = 0.0f;
+        int ca1 = 0, cb1 = 0;
+        if (REF) {
+            fla1 = floorf(a1);
+            flb1 = floorf(b1);
+            slab_segments(a0, a1, fla0, fla1, b0, b1, flb0, flb1, w, s);
+        } else {
+            ca1 = cell_floor(a1);
+            cb1 = cell_floor(b1);
+            fra1 = cell_frac(a1);
+            frb1 = cell_frac(b1);
+            slab_exact<MODE == 2>(ca0, ca1, fra0, cb0, cb1, frb0, r, pia, pib, s);
+        }
         const bool act = mine && (t >= r.s0) && (t < r.s1) && s.ok;
         const int o1 = (s.a1 + __mul24(s.b1, w.sb) + tb) * 4;
         const int o2 = (s.a2 + __mul24(s.b2, w.sb) + tb) * 4;
@@ -341,7 +374,12 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
         } else {
             acc += (float)((int)u1 + (int)u2 + (int)u3);
         }
-        a0 = a1; b0 = b1; fla0 = fla1; flb0 = flb1;
+        a0 = a1; b0 = b1;
+        if (REF) {
+            fla0 = fla1; flb0 = flb1;
+        } else {
+            ca0 = ca1; cb0 = cb1; fra0 = fra1; frb0 = frb1;
+        }
         tb += w.st;
     }
     return acc;
@@ -545,7 +583,12 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
             const float a1 = fmaf(r[j].ka, tf, r[j].Ac), b1 = fmaf(r[j].kb, tf, r[j].Bc);
             const float fla1 = floorf(a1), flb1 = floorf(b1);
             Seg s;
-            slab_segments<REF>(a0[j], a1, fla0[j], fla1, b0[j], b1, flb0[j], flb1, r[j], w, s);
+            if (REF)
+                slab_segments(a0[j], a1, fla0[j], fla1, b0[j], b1, flb0[j], flb1, w, s);
+            else
+                slab_exact<false>(cell_floor(a0[j]), cell_floor(a1), cell_frac(a0[j]), cell_floor(b0[j]),
+                                  cell_floor(b1), cell_frac(b0[j]), r[j], r[j].pa * r[j].inv_ka,
+                                  r[j].pb * r[j].inv_kb, s);
             const bool act = mine[j] && (t >= r[j].s0) && (t < r[j].s1) && s.ok;
             const int ca[3] = {s.a1, s.a2, s.a3};
             const int cb[3] = {s.b1, s.b2, s.b3};
@@ -942,28 +985,34 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             float a0 = fmaf(r.ka, tf, r.Ac), bb0 = fmaf(r.kb, tf, r.Bc);
             float fla0 = floorf(a0), flb0 = floorf(bb0);
             if (!REF) {
-                // Exact rule with the slab's entry cell carried from the previous
-                // slab: its integer index, LDS offset and range test are computed
-                // once per slab, not once per segment.  Same arithmetic as
-                // slab_segments<false>.
-                int ca0 = (int)fla0, cb0 = (int)flb0;
+                // Exact rule (slab_exact, the forward's arithmetic) with the
+                // slab's entry cell carried from the previous slab: its index,
+                // fraction, range test and LDS offset E are computed once per
+                // slab.  The exit cell's offset is E + dA + dB, the middle
+                // piece's E + dA or E + dB, and the next entry is exit + St.
+                int ca0 = cell_floor(a0), cb0 = cell_floor(bb0);
+                float fra0 = cell_frac(a0), frb0 = cell_frac(bb0);
+                const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
                 const int qs = FIX ? (int)sv : 0;               // |sv| <= 2^30
                 const int hi = (FIX && sv < 0.0f) ? -1 : 0;
                 int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
+                int E = A0 + B0 + row;
                 bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
 #pragma unroll 2
                 for (int t = t0; t < t1; ++t) {
                     tf += 1.0f;
                     const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
-                    const float fla1 = floorf(a1), flb1 = floorf(bb1);
-                    const int ca1 = (int)fla1, cb1 = (int)flb1;
-                    const float fa = clamp01((fla0 + r.pa - a0) * r.inv_ka);
-                    const float fb = clamp01((flb0 + r.pb - bb0) * r.inv_kb);
+                    const int ca1 = cell_floor(a1), cb1 = cell_floor(bb1);
+                    const float fra1 = cell_frac(a1), frb1 = cell_frac(bb1);
+                    const float fa = cross_frac(fra0, r.inv_ka, pia);
+                    const float fb = cross_frac(frb0, r.inv_kb, pib);
                     const bool afirst = fa < fb;
                     const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
                     const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
+                    const int dA = A1 - A0, dB = B1 - B0;
+                    const int X = E + dA + dB;
                     const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
-                    const bool i2 = afirst ? (ia1 && ib0) : (ia0 && ib1);
+                    const bool i2 = (afirst & ia1 & ib0) | (!afirst & ia0 & ib1);
                     if constexpr (FIX) {
                         // the three pieces as differences of the truncated
                         // prefix sums f1 sv, f2 sv, sv: they add up to the
@@ -972,26 +1021,24 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                         // word is the ray's constant
                         const int q1 = (int)(f1 * sv), q2 = (int)(f2 * sv);
                         const int d2 = q2 - q1, d3 = qs - q2;
-                        if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, A0 + B0 + row, q1, hi);
-                        if (i2 && d2 != 0) lds_deposit_q(ab, (afirst ? A1 + B0 : A0 + B1) + row, d2, hi);
-                        if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, A1 + B1 + row, d3, hi);
+                        if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, E, q1, hi);
+                        if (i2 && d2 != 0) lds_deposit_q(ab, E + (afirst ? dA : dB), d2, hi);
+                        if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, hi);
                     } else {
                         const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
-                        if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, A0 + B0 + row, w1 * sv);
-                        if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, (afirst ? A1 + B0 : A0 + B1) + row, w2 * sv);
-                        if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, A1 + B1 + row, w3 * sv);
+                        if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, E, w1 * sv);
+                        if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, E + (afirst ? dA : dB), w2 * sv);
+                        if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, X, w3 * sv);
                     }
-                    a0 = a1;
-                    bb0 = bb1;
-                    fla0 = fla1;
-                    flb0 = flb1;
                     ca0 = ca1;
                     cb0 = cb1;
+                    fra0 = fra1;
+                    frb0 = frb1;
                     A0 = A1;
                     B0 = B1;
+                    E = X + St;
                     ia0 = ia1;
                     ib0 = ib1;
-                    row += St;
                 }
                 continue;
             }
@@ -1000,7 +1047,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
                 const float fla1 = floorf(a1), flb1 = floorf(bb1);
                 Seg sg;
-                slab_segments<REF>(a0, a1, fla0, fla1, bb0, bb1, flb0, flb1, r, w, sg);
+                slab_segments(a0, a1, fla0, fla1, bb0, bb1, flb0, flb1, w, sg);
                 if (sg.ok) {
                     const int ca[3] = {sg.a1, sg.a2, sg.a3};
                     const int cb[3] = {sg.b1, sg.b2, sg.b3};
