@@ -146,7 +146,19 @@ __device__ __forceinline__ void clip_axis(double k, double inv_k, double A, doub
     tout = fmin(tout, t1);
 }
 
-// FP64 ray set-up (three divisions and one square root per ray).
+// 1/x in FP64 from the hardware reciprocal and two Newton steps (a few ulp at
+// most, against ~10 instructions for a correctly rounded division).  The
+// forward and the adjoint share this set-up, so they see identical rays.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// FP64 ray set-up (three reciprocals; the length per slab in FP32).
 // 'ref' selects the reference's x/y-only dominance rule.
 __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
                           unsigned flags, bool adjoint, Ray& r)
@@ -177,10 +189,10 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     if (d[m] == 0.0) return;
     const int a = (m == 0) ? 1 : 0;
     const int b = (m == 2) ? 1 : 2;
-    const double inv_dm = 1.0 / d[m];
+    const double inv_dm = rcp64(d[m]);
     const double ka = d[a] * inv_dm, kb = d[b] * inv_dm;
-    const double ika = (d[a] != 0.0) ? d[m] / d[a] : 0.0;
-    const double ikb = (d[b] != 0.0) ? d[m] / d[b] : 0.0;
+    const double ika = (d[a] != 0.0) ? d[m] * rcp64(d[a]) : 0.0;
+    const double ikb = (d[b] != 0.0) ? d[m] * rcp64(d[b]) : 0.0;
     const double Aa = uS[a] - ka * uS[m], Ab = uS[b] - kb * uS[m];
     // The exact walk clips to the volume box.  The reference rule can credit an
     // in-range voxel from a slab whose exact segment lies just outside the box
@@ -211,7 +223,7 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     r.pa = ka > 0.0 ? 1.0f : 0.0f;
     r.pb = kb > 0.0 ? 1.0f : 0.0f;
     const double qa = ka * V.vs[a] * V.inv_vs[m], qb = kb * V.vs[b] * V.inv_vs[m];
-    double sc = V.vs[m] * sqrt(1.0 + qa * qa + qb * qb);
+    double sc = V.vs[m] * (double)sqrtf((float)(1.0 + qa * qa + qb * qb));   // rounded to FP32 below
     if (flags & NEFCT_F_INV_L2) sc = sc / (xd * xd + yd * yd + zd * zd);
     r.scale = (float)sc;
 }
