@@ -99,6 +99,20 @@ __device__ __forceinline__ float cross_frac(float frac, float inv_k, float pa_in
     return clamp01(fmaf(frac, -inv_k, pa_inv_k));
 }
 
+// Rays with a transverse slope within 2^-8 of +-1 (views near 45 degrees).
+// Their FP32 face coordinates advance by up to |k| + 2 ulp per slab, so a
+// face coordinate can round across a second cell boundary (with |x| < 2^13
+// one ulp is <= 2^-10) and the cells taken from floor() would skip a cell.
+// These rays take the cell sequence from the crossings instead: one step per
+// crossing, the in-cell fraction measured from the stepped cell (it may leave
+// [0, 1] by a rounding error, which the clamp of cross_frac absorbs: a
+// coordinate past the boundary crosses at once, one short of it does not).
+constexpr float K_NEAR_ONE = 1.0f - 1.0f / 256.0f;
+__device__ __forceinline__ bool near_unit_slope(const Ray& r)
+{
+    return fabsf(r.ka) > K_NEAR_ONE || fabsf(r.kb) > K_NEAR_ONE;
+}
+
 // floor / ceil of v as an int clamped to [lo, hi] before the conversion, so
 // huge or non-finite values (slopes near zero) cannot overflow; NaN gives lo
 __device__ __forceinline__ int floor_clamp(float v, int lo, int hi)
@@ -276,6 +290,19 @@ struct Seg {
     bool ok;     // reference outer range test (siddon.cu.cc:80,112,132)
 };
 
+// the three pieces of a slab from its crossing fractions and cells
+__device__ __forceinline__ void slab_pieces(int ca0, int ca1, float fa, int cb0, int cb1, float fb, Seg& s)
+{
+    const bool afirst = fa < fb;
+    const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
+    s.a1 = ca0; s.b1 = cb0;
+    s.a2 = afirst ? ca1 : ca0;
+    s.b2 = afirst ? cb0 : cb1;
+    s.a3 = ca1; s.b3 = cb1;
+    s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
+    s.ok = true;
+}
+
 template <bool GUARD>
 __device__ __forceinline__ void slab_exact(int ca0, int ca1, float fra0, int cb0, int cb1, float frb0, const Ray& r,
                                            float pia, float pib, Seg& s)
@@ -286,14 +313,21 @@ __device__ __forceinline__ void slab_exact(int ca0, int ca1, float fra0, int cb0
         fa = (ca1 != ca0) ? fa : 1.0f;
         fb = (cb1 != cb0) ? fb : 1.0f;
     }
-    const bool afirst = fa < fb;
-    const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
-    s.a1 = ca0; s.b1 = cb0;
-    s.a2 = afirst ? ca1 : ca0;
-    s.b2 = afirst ? cb0 : cb1;
-    s.a3 = ca1; s.b3 = cb1;
-    s.w1 = f1; s.w2 = f2 - f1; s.w3 = 1.0f - f2;
-    s.ok = true;
+    slab_pieces(ca0, ca1, fa, cb0, cb1, fb, s);
+}
+
+// near_unit_slope rays: the cells step with the crossings; returns the exit
+// cells and the in-cell fractions at the slab's far face
+__device__ __forceinline__ void slab_step(int ca0, float fra0, int cb0, float frb0, float a1, float b1, const Ray& r,
+                                          float pia, float pib, int& ca1, float& fra1, int& cb1, float& frb1, Seg& s)
+{
+    const float fa = (r.ka != 0.0f) ? cross_frac(fra0, r.inv_ka, pia) : 1.0f;
+    const float fb = (r.kb != 0.0f) ? cross_frac(frb0, r.inv_kb, pib) : 1.0f;
+    ca1 = ca0 + ((fa < 1.0f) ? ((r.ka > 0.0f) ? 1 : -1) : 0);
+    cb1 = cb0 + ((fb < 1.0f) ? ((r.kb > 0.0f) ? 1 : -1) : 0);
+    fra1 = a1 - (float)ca1;
+    frb1 = b1 - (float)cb1;
+    slab_pieces(ca0, ca1, fa, cb0, cb1, fb, s);
 }
 
 __device__ __forceinline__ void slab_segments(float a0, float a1, float fla0, float fla1, float b0, float b1,
@@ -329,7 +363,7 @@ __device__ __forceinline__ bool cell_in(const Walk& w, int ca, int cb)
 // The three gathers of a slab are issued as three predicated loads whose
 // results are consumed only after all of them, so a wavefront has all of a
 // slab's loads (two slabs' with the unroll) in flight at once.
-template <int MODE, bool REF, bool CHECKED>
+template <int MODE, bool REF, bool CHECKED, bool STEP = false>
 __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Walk& w,
                           __amdgpu_buffer_rsrc_t rs, float val)
 {
@@ -354,6 +388,8 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
             fla1 = floorf(a1);
             flb1 = floorf(b1);
             slab_segments(a0, a1, fla0, fla1, b0, b1, flb0, flb1, w, s);
+        } else if (STEP) {
+            slab_step(ca0, fra0, cb0, frb0, a1, b1, r, pia, pib, ca1, fra1, cb1, frb1, s);
         } else {
             ca1 = cell_floor(a1);
             cb1 = cell_floor(b1);
@@ -447,11 +483,24 @@ __device__ float project_all_dominances(const Launch& P, const Ray& r, bool insi
         float* base = (d == 0) ? P.Ly.yf : P.Ly.xf;
         const uint32_t nbytes = (d == 0) ? P.Ly.yf_bytes : P.Ly.xf_bytes;
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, nbytes);
-        float v;
-        if (d == 2)
-            v = walk_ray<MODE, REF, true>(r, mine, lo, hi, w, rs, val);
-        else
-            v = walk_ray<MODE, REF, false>(r, mine, lo, hi, w, rs, val);
+        // rays of near-unit slope walk the stepped-cell rule (a few per view
+        // near 45 degrees); the branch costs nothing when no lane has one
+        const bool stp = !REF && mine && near_unit_slope(r);
+        float v = 0.0f;
+        if (__any(mine && !stp)) {
+            if (d == 2)
+                v = walk_ray<MODE, REF, true>(r, mine && !stp, lo, hi, w, rs, val);
+            else
+                v = walk_ray<MODE, REF, false>(r, mine && !stp, lo, hi, w, rs, val);
+        }
+        if (!REF && __any(stp)) {
+            float v2;
+            if (d == 2)
+                v2 = walk_ray<MODE, REF, true, true>(r, stp, lo, hi, w, rs, val);
+            else
+                v2 = walk_ray<MODE, REF, false, true>(r, stp, lo, hi, w, rs, val);
+            if (stp) v = v2;
+        }
         if (mine) acc = v;
     }
     return acc;
@@ -568,7 +617,8 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
                                          int hi, const Walk& w, __amdgpu_buffer_rsrc_t rs, float (*win)[WIN_B][WIN_A],
                                          const float (&bnd)[8], int wave, int lane)
 {
-    float a0[AR], b0[AR], fla0[AR], flb0[AR];
+    float a0[AR], b0[AR], fla0[AR], flb0[AR], fra0[AR], frb0[AR];
+    int ca0[AR], cb0[AR];
 #pragma unroll
     for (int j = 0; j < AR; ++j) {
         const float tf = (float)(lo - r[j].tc);
@@ -576,6 +626,10 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
         b0[j] = fmaf(r[j].kb, tf, r[j].Bc);
         fla0[j] = floorf(a0[j]);
         flb0[j] = floorf(b0[j]);
+        ca0[j] = cell_floor(a0[j]);
+        cb0[j] = cell_floor(b0[j]);
+        fra0[j] = cell_frac(a0[j]);
+        frb0[j] = cell_frac(b0[j]);
     }
     const float amn = bnd[0], amx = bnd[1], kamn = bnd[2], kamx = bnd[3];
     const float bmn = bnd[4], bmx = bnd[5], kbmn = bnd[6], kbmx = bnd[7];
@@ -595,12 +649,22 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
             const float a1 = fmaf(r[j].ka, tf, r[j].Ac), b1 = fmaf(r[j].kb, tf, r[j].Bc);
             const float fla1 = floorf(a1), flb1 = floorf(b1);
             Seg s;
-            if (REF)
+            int ca1 = 0, cb1 = 0;
+            float fra1 = 0.0f, frb1 = 0.0f;
+            if (REF) {
                 slab_segments(a0[j], a1, fla0[j], fla1, b0[j], b1, flb0[j], flb1, w, s);
-            else
-                slab_exact<false>(cell_floor(a0[j]), cell_floor(a1), cell_frac(a0[j]), cell_floor(b0[j]),
-                                  cell_floor(b1), cell_frac(b0[j]), r[j], r[j].pa * r[j].inv_ka,
-                                  r[j].pb * r[j].inv_kb, s);
+            } else {
+                const float pia = r[j].pa * r[j].inv_ka, pib = r[j].pb * r[j].inv_kb;
+                if (near_unit_slope(r[j])) {
+                    slab_step(ca0[j], fra0[j], cb0[j], frb0[j], a1, b1, r[j], pia, pib, ca1, fra1, cb1, frb1, s);
+                } else {
+                    ca1 = cell_floor(a1);
+                    cb1 = cell_floor(b1);
+                    fra1 = cell_frac(a1);
+                    frb1 = cell_frac(b1);
+                    slab_exact<false>(ca0[j], ca1, fra0[j], cb0[j], cb1, frb0[j], r[j], pia, pib, s);
+                }
+            }
             const bool act = mine[j] && (t >= r[j].s0) && (t < r[j].s1) && s.ok;
             const int ca[3] = {s.a1, s.a2, s.a3};
             const int cb[3] = {s.b1, s.b2, s.b3};
@@ -622,6 +686,10 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
             b0[j] = b1;
             fla0[j] = fla1;
             flb0[j] = flb1;
+            ca0[j] = ca1;
+            cb0[j] = cb1;
+            fra0[j] = fra1;
+            frb0[j] = frb1;
         }
         __syncthreads();
         for (int lb = wave; lb < span_b; lb += BY) {
@@ -1002,56 +1070,79 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 // fraction, range test and LDS offset E are computed once per
                 // slab.  The exit cell's offset is E + dA + dB, the middle
                 // piece's E + dA or E + dB, and the next entry is exit + St.
-                int ca0 = cell_floor(a0), cb0 = cell_floor(bb0);
-                float fra0 = cell_frac(a0), frb0 = cell_frac(bb0);
+                // near_unit_slope rays step their cells with the crossings
+                // (slab_step), as the forward walks them.
                 const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
                 const int qs = FIX ? (int)sv : 0;               // |sv| <= 2^30
                 const int hi = (FIX && sv < 0.0f) ? -1 : 0;
-                int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
-                int E = A0 + B0 + row;
-                bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
+                auto walk = [&](auto step_tag) {
+                    constexpr bool STEP = decltype(step_tag)::value;
+                    float tw = tf;
+                    int ca0 = cell_floor(a0), cb0 = cell_floor(bb0);
+                    float fra0 = cell_frac(a0), frb0 = cell_frac(bb0);
+                    int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
+                    int E = A0 + B0 + row;
+                    bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
 #pragma unroll 2
-                for (int t = t0; t < t1; ++t) {
-                    tf += 1.0f;
-                    const float a1 = fmaf(r.ka, tf, r.Ac), bb1 = fmaf(r.kb, tf, r.Bc);
-                    const int ca1 = cell_floor(a1), cb1 = cell_floor(bb1);
-                    const float fra1 = cell_frac(a1), frb1 = cell_frac(bb1);
-                    const float fa = cross_frac(fra0, r.inv_ka, pia);
-                    const float fb = cross_frac(frb0, r.inv_kb, pib);
-                    const bool afirst = fa < fb;
-                    const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
-                    const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
-                    const int dA = A1 - A0, dB = B1 - B0;
-                    const int X = E + dA + dB;
-                    const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
-                    const bool i2 = (afirst & ia1 & ib0) | (!afirst & ia0 & ib1);
-                    if constexpr (FIX) {
-                        // the three pieces as differences of the truncated
-                        // prefix sums f1 sv, f2 sv, sv: they add up to the
-                        // slab's whole contribution exactly, and all share
-                        // the ray's sign, so zeros are skipped and the high
-                        // word is the ray's constant
-                        const int q1 = (int)(f1 * sv), q2 = (int)(f2 * sv);
-                        const int d2 = q2 - q1, d3 = qs - q2;
-                        if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, E, q1, hi);
-                        if (i2 && d2 != 0) lds_deposit_q(ab, E + (afirst ? dA : dB), d2, hi);
-                        if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, hi);
-                    } else {
-                        const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
-                        if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, E, w1 * sv);
-                        if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, E + (afirst ? dA : dB), w2 * sv);
-                        if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, X, w3 * sv);
+                    for (int t = t0; t < t1; ++t) {
+                        tw += 1.0f;
+                        const float a1 = fmaf(r.ka, tw, r.Ac), bb1 = fmaf(r.kb, tw, r.Bc);
+                        int ca1, cb1;
+                        float fra1, frb1, fa, fb;
+                        if constexpr (STEP) {
+                            fa = (r.ka != 0.0f) ? cross_frac(fra0, r.inv_ka, pia) : 1.0f;
+                            fb = (r.kb != 0.0f) ? cross_frac(frb0, r.inv_kb, pib) : 1.0f;
+                            ca1 = ca0 + ((fa < 1.0f) ? ((r.ka > 0.0f) ? 1 : -1) : 0);
+                            cb1 = cb0 + ((fb < 1.0f) ? ((r.kb > 0.0f) ? 1 : -1) : 0);
+                            fra1 = a1 - (float)ca1;
+                            frb1 = bb1 - (float)cb1;
+                        } else {
+                            ca1 = cell_floor(a1);
+                            cb1 = cell_floor(bb1);
+                            fra1 = cell_frac(a1);
+                            frb1 = cell_frac(bb1);
+                            fa = cross_frac(fra0, r.inv_ka, pia);
+                            fb = cross_frac(frb0, r.inv_kb, pib);
+                        }
+                        const bool afirst = fa < fb;
+                        const float f1 = afirst ? fa : fb, f2 = afirst ? fb : fa;
+                        const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
+                        const int dA = A1 - A0, dB = B1 - B0;
+                        const int X = E + dA + dB;
+                        const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
+                        const bool i2 = (afirst & ia1 & ib0) | (!afirst & ia0 & ib1);
+                        if constexpr (FIX) {
+                            // the three pieces as differences of the truncated
+                            // prefix sums f1 sv, f2 sv, sv: they add up to the
+                            // slab's whole contribution exactly, and all share
+                            // the ray's sign, so zeros are skipped and the high
+                            // word is the ray's constant
+                            const int q1 = (int)(f1 * sv), q2 = (int)(f2 * sv);
+                            const int d2 = q2 - q1, d3 = qs - q2;
+                            if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, E, q1, hi);
+                            if (i2 && d2 != 0) lds_deposit_q(ab, E + (afirst ? dA : dB), d2, hi);
+                            if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, hi);
+                        } else {
+                            const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
+                            if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, E, w1 * sv);
+                            if (i2 && w2 > 0.0f) lds_deposit<FIX>(ab, E + (afirst ? dA : dB), w2 * sv);
+                            if (ia1 && ib1 && w3 > 0.0f) lds_deposit<FIX>(ab, X, w3 * sv);
+                        }
+                        ca0 = ca1;
+                        cb0 = cb1;
+                        fra0 = fra1;
+                        frb0 = frb1;
+                        A0 = A1;
+                        B0 = B1;
+                        E = X + St;
+                        ia0 = ia1;
+                        ib0 = ib1;
                     }
-                    ca0 = ca1;
-                    cb0 = cb1;
-                    fra0 = fra1;
-                    frb0 = frb1;
-                    A0 = A1;
-                    B0 = B1;
-                    E = X + St;
-                    ia0 = ia1;
-                    ib0 = ib1;
-                }
+                };
+                if (near_unit_slope(r))
+                    walk(std::true_type{});
+                else
+                    walk(std::false_type{});
                 continue;
             }
             for (int t = t0; t < t1; ++t) {

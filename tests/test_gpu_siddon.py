@@ -257,6 +257,49 @@ def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
         assert _relerr(got_r, ref_r) < 1e-5, (algo, name, "reference mode", _relerr(got_r, ref_r))
 
 
+@pytest.mark.parametrize("shift", [0.0, 1.0 / 256])
+def test_boundary_aligned_rays(oracle, shift):
+    """Rays on (or within 1/256 voxel of) voxel faces: views at multiples of
+    45 degrees (the float32 angles leave transverse slopes of 0 or ~1e-8) and
+    detector cells whose rays pass through the isocentre planes x = 0, y = 0,
+    z = 0 (even sizes, no offsets, one voxel per cell at the isocentre), with
+    the volume shifted by 'shift' voxels along each axis.  Stresses the zero-
+    and near-zero-slope paths, the block overlap margin and the crossing
+    fraction at cell boundaries.  Checks: the transpose (dot test, 1e-6) and
+    the agreement of every adjoint kernel with the default one (same weights,
+    1e-6) always; the FP64 oracle to 1e-5 when shifted.  Unshifted, a ray lying
+    in a face has no unique exact answer (its length belongs to either
+    neighbour, decided by rounding), so there the oracle only bounds the
+    result loosely (1e-2)."""
+    nh = _nh()
+    from nefct_hip import ConeGeometry, VolumeGeometry, _lib
+    vol = VolumeGeometry((32, 32, 32), (shift, shift, shift), (32.0, 32.0, 32.0))
+    det = ConeGeometry.flat(100.0, 200.0, 80, 2.0, 80, 2.0, ai=0.0, bi=0.0)
+    ang = (np.arange(8) * (math.pi / 4)).astype(np.float32)
+    from nefct_hip.phantom import ellipsoid_phantom
+    img = ellipsoid_phantom(vol, 7, 6)
+    y = np.random.default_rng(3).uniform(0, 1, size=(ang.size, det.nb, det.na)).astype(np.float32)
+    yt = torch.tensor(y, device=DEV)
+    base = nh.backproject(yt, vol, det, torch.tensor(ang)).cpu().numpy()
+    for fl in (_lib.F_TILE_64x32x8, _lib.F_TILE_16x16x32 | _lib.F_LANE_STRIDE2, _lib.F_ADJ_FLOAT_ATOMIC,
+               _lib.F_ADJ_WINDOW):
+        bp = nh.backproject(yt, vol, det, torch.tensor(ang), flags=fl).cpu().numpy()
+        assert _relerr(bp, base) < 1e-6, (fl, _relerr(bp, base))
+    x = torch.randn(vol.shape_zyx, generator=torch.Generator().manual_seed(5)).to(DEV)
+    yr = torch.randn((ang.size, det.nb, det.na), generator=torch.Generator().manual_seed(6)).to(DEV)
+    lhs = torch.sum(nh.project(x, vol, det, torch.tensor(ang)).double() * yr.double()).item()
+    rhs = torch.sum(x.double() * nh.backproject(yr, vol, det, torch.tensor(ang)).double()).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs), (lhs, rhs)
+    ov = oracle.vol_struct(vol.grid, vol.center, vol.size)
+    od = oracle.det_struct(det.cyl, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi)
+    got = nh.project(img.to(DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    ref = oracle.exact_fwd(ov, od, ang.astype(np.float64), img.double().numpy())
+    ref_bp = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
+    tol = 1e-5 if shift else 1e-2
+    assert _relerr(got, ref) < tol, _relerr(got, ref)
+    assert _relerr(base, ref_bp) < tol, _relerr(base, ref_bp)
+
+
 def test_deterministic_adjoint_is_bitwise_reproducible():
     nh = _nh()
     from nefct_hip import _lib
