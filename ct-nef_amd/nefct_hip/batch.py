@@ -1,7 +1,7 @@
 """Batched Siddon pair over many problems of uneven size.
 
 One launch per direction covers every problem: the forward's workgroups are
-64x4-ray tasks (one view, neighbouring views dealt to the same XCD) and the
+16x16-ray tasks (one view, neighbouring views dealt to the same XCD) and the
 adjoint's 32x32x16 voxel blocks, numbered across all problems, so small
 problems do not leave the GPU idle and there is one launch instead of one per
 problem (the reference loops

@@ -152,7 +152,7 @@ typedef struct {
     int na, nb;
     float da, ai, db, bi;
     /* filled by nefct_batch_plan */
-    long long tile_begin;   /* first forward task (64x4 rays of one view; views padded to groups of 16) */
+    long long tile_begin;   /* first forward task (16x16 rays of one view; views padded to groups of 16) */
     long long block_begin;  /* first adjoint task (one 32x32x16 voxel block) */
     long long ws_offset;    /* byte offset of the problem's workspace region */
     int tiles_a, tiles_b;
@@ -166,7 +166,7 @@ int nefct_batch_plan(nefct_problem* problems, int nprob, unsigned flags, size_t*
                      long long* total_tiles, long long* total_blocks);
 
 /* Forward projection of every problem: one trig/launch-record pass, the
- * padded volume copies, then one launch over the 64x4-ray tasks of all
+ * padded volume copies, then one launch over the 16x16-ray tasks of all
  * problems, dealt to the XCDs in contiguous ranges.  'host_problems' is the
  * planned table, 'device_problems' a device copy of it. */
 int nefct_batch_project(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
