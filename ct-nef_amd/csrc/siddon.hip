@@ -892,11 +892,12 @@ __device__ __forceinline__ void lds_deposit_q(char* base, int byte_off, int q, i
 }
 
 __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
-                                              const int hi[3], int& ia0, int& ia1, int& ib0, int& ib1)
+                                              const int hi[3], bool tight, int& ia0, int& ia1, int& ib0,
+                                              int& ib1)
 {
     // Fractional detector indices of the 8 corners (flat: a = t SDD / U,
     // b = z SDD / U; cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)),
-    // with a margin of 1 cell.  Lane l computes corner l & 7 and the groups of
+    // with the margin chosen below.  Lane l computes corner l & 7 and the groups of
     // 8 lanes reduce by butterfly, so every lane ends with the same bounds;
     // readfirstlane makes them scalar.
     const int q = threadIdx.x & 7;
@@ -930,6 +931,23 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
         ia0 = 0; ia1 = G.na - 1; ib0 = 0; ib1 = G.nb - 1;
         return;
     }
+    if (tight && !G.cyl) {
+        // Flat detector, exact rule: the block's projection is the convex
+        // hull of its corners' (a linear-fractional map), and cell i's ray
+        // goes through fractional index i, so the cells whose rays meet the
+        // block are ceil(min) .. floor(max).  The margin (1/8 cell) covers
+        // the FP32 rounding of these indices and of the walk's coordinates
+        // many times over.
+        constexpr float M = 0.125f;
+        ia0 = __builtin_amdgcn_readfirstlane(max(0, ceil_clamp(fa_mn - M, -2, G.na + 1)));
+        ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, floor_clamp(fa_mx + M, -2, G.na + 1)));
+        ib0 = __builtin_amdgcn_readfirstlane(max(0, ceil_clamp(fb_mn - M, -2, G.nb + 1)));
+        ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, floor_clamp(fb_mx + M, -2, G.nb + 1)));
+        return;
+    }
+    // one cell of margin: on a cylindrical detector the row coordinate of an
+    // edge point can exceed its corners'; the reference rule can credit a
+    // cell next to its exact segment
     ia0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fa_mn, -2, G.na + 1) - 1));
     ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, ceil_clamp(fa_mx, -2, G.na + 1) + 1));
     ib0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fb_mn, -2, G.nb + 1) - 1));
@@ -957,7 +975,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     for (int iv = 0; iv < P.nv; ++iv) {
         const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
         int ia0, ia1, ib0, ib1;
-        shadow_of_box(P.V, P.G, c, s, b0, b1, ia0, ia1, ib0, ib1);
+        shadow_of_box(P.V, P.G, c, s, b0, b1, !REF, ia0, ia1, ib0, ib1);
         if (ia1 < ia0 || ib1 < ib0) continue;
         // The shadow's W x H detector cells are numbered column by column
         // and dealt out densely: wavefront w takes cells w*64 + lane +
