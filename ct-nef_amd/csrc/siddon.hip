@@ -81,9 +81,12 @@ __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed
 
 // Cell of a walk coordinate and the position inside it: floor(x) as an int
 // and x - floor(x) (v_fract_f32).  The coordinates are bounded (|x| < 2^20).
-// (v_cvt_flr_i32_f32 would give the int in one instruction, but as inline
-// assembly it is treated as convergent, which stops the walk loops from being
-// unrolled; the unroll is worth more.)
+// This file's device code is compiled without NaN semantics
+// (-Xarch_device -fno-honor-nans), which lets (int)floorf(x) become one
+// v_cvt_flr_i32_f32 (~5 % of the walk loops).  Nothing here relies on NaN
+// arithmetic: tests for non-finite data work on the bits (fixed_scale,
+// the fixed-point write-out, the window adjoint's flush), and walk and
+// set-up values are finite by construction.
 __device__ __forceinline__ int cell_floor(float x) { return (int)floorf(x); }
 __device__ __forceinline__ float cell_frac(float x) { return __builtin_amdgcn_fractf(x); }
 
@@ -114,7 +117,7 @@ __device__ __forceinline__ bool near_unit_slope(const Ray& r)
 }
 
 // floor / ceil of v as an int clamped to [lo, hi] before the conversion, so
-// huge or non-finite values (slopes near zero) cannot overflow; NaN gives lo
+// huge values (slopes near zero) cannot overflow; v is finite
 __device__ __forceinline__ int floor_clamp(float v, int lo, int hi)
 {
     return (int)fminf(fmaxf(floorf(v), (float)lo), (float)hi);
@@ -695,7 +698,7 @@ __device__ __forceinline__ void adj_walk(const Ray (&r)[AR], const bool (&mine)[
         for (int lb = wave; lb < span_b; lb += BY) {
             for (int la = lane; la < span_a; la += BX) {
                 const float v = cur[lb][la];
-                if (v != 0.0f) {
+                if ((__float_as_uint(v) << 1) != 0u) {   // not +-0 (NaN included)
                     cur[lb][la] = 0.0f;
                     const int ca = oa + la, cb = ob + lb;
                     if (!(CHECKED || REF) || cell_in(w, ca, cb))
@@ -860,9 +863,9 @@ using acc_t = typename std::conditional<FIX, unsigned long long, float>::type;
 // bits) and the bound on the per-slab ray length; 0 marks non-finite input.
 __device__ __forceinline__ float fixed_scale(unsigned maxbits, float sbound)
 {
-    const float mx = __uint_as_float(maxbits);
-    if (!(mx <= 3.0e38f)) return 0.0f;          // Inf or NaN somewhere
-    const float big = mx * sbound;
+    // Inf or NaN somewhere (tested on the bits: NaN and Inf sort above 3e38)
+    if (maxbits > __builtin_bit_cast(unsigned, 3.0e38f)) return 0.0f;
+    const float big = __uint_as_float(maxbits) * sbound;
     if (!(big > 0.0f)) return 1.0f;             // all zero
     int e;
     frexpf(1073741824.0f / big, &e);            // 2^30 / big = m 2^e, m in [0.5, 1)
@@ -1048,22 +1051,19 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             // tighten [t0, t1) to the slabs where the ray is within the margin
             // box, with one slab of slack on each side (the clamps keep the
             // result while bounding the conversion: t0 and t1 already lie in
-            // [b0[m], b1[m]]; a NaN estimate, 0 * inf, leaves the range as is)
-            if (r.ka != 0.0f) {
+            // [b0[m], b1[m]]).  A slope below 1e-30 leaves the range as is;
+            // otherwise 1/k < ~1e30 and the estimates are finite.
+            if (fabsf(r.ka) >= 1e-30f) {
                 float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                if (ta == ta && tb == tb) {
-                    t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                    t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
-                }
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
             }
-            if (r.kb != 0.0f) {
+            if (fabsf(r.kb) >= 1e-30f) {
                 float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                if (ta == ta && tb == tb) {
-                    t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                    t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
-                }
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
             }
             if (t0 >= t1) continue;
             const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
@@ -1188,11 +1188,18 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         }
     }
     __syncthreads();
-    const float inv_s = FIX ? (S > 0.0f ? 1.0f / S : __builtin_nanf("")) : 1.0f;   // S is a power of two
+    // S is a power of two, or 0 for a non-finite sinogram: then every voxel is
+    // written as a NaN bit pattern (an integer store)
+    const bool poisoned = FIX && S == 0.0f;
+    const float inv_s = FIX ? 1.0f / (poisoned ? 1.0f : S) : 1.0f;
     for (int i = tid; i < TX * TY * TZ; i += 64 * NW) {
         const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
         if (lx < ext[0] && ly < ext[1] && lz < ext[2]) {
             const size_t o = (size_t)(b0[0] + lx) + (size_t)nx * ((size_t)(b0[1] + ly) + (size_t)ny * (size_t)(b0[2] + lz));
+            if (poisoned) {
+                reinterpret_cast<unsigned*>(image)[o] = 0x7fc00000u;
+                continue;
+            }
             float v;
             if (FIX)
                 v = (float)((double)(long long)acc.at(lx, ly, lz) * (double)inv_s);
