@@ -1038,9 +1038,14 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             // if the walk's position there reaches the block's range; the
             // walk's FP32 positions are within 2^-24 * 512 cells of the exact
             // line and the slab estimates below within 600 * 2^-23 * |1/k|
-            // slabs, both far inside MARGIN (and its MARGIN / |k| slabs) plus
-            // the one slab of slack on each side.
+            // slabs, both far inside MARGIN (and its MARGIN / |k| slabs).
+            // Slab t (positions [t, t + 1]) meets the widened range [u, v]
+            // iff floor(u) <= t < ceil(v) up to ties, where the slab only
+            // touches a widened face.  The reference rule can credit a
+            // neighbour of its exact cell, so it keeps one slab of slack on
+            // each side.
             constexpr float MARGIN = 1.0f / 64.0f;
+            constexpr int SLACK = REF ? 1 : 0;
             const float fa0 = fmaf(r.ka, (float)(t0 - r.tc), r.Ac), fa1 = fmaf(r.ka, (float)(t1 - r.tc), r.Ac);
             const float fb0 = fmaf(r.kb, (float)(t0 - r.tc), r.Bc), fb1 = fmaf(r.kb, (float)(t1 - r.tc), r.Bc);
             const int ha = P.V.n[a] >> 1, hb = P.V.n[b] >> 1;
@@ -1049,21 +1054,21 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             if (fmaxf(fa0, fa1) < alo || fminf(fa0, fa1) > ahi || fmaxf(fb0, fb1) < blo || fminf(fb0, fb1) > bhi)
                 continue;
             // tighten [t0, t1) to the slabs where the ray is within the margin
-            // box, with one slab of slack on each side (the clamps keep the
+            // box, with SLACK slabs on each side (the clamps keep the
             // result while bounding the conversion: t0 and t1 already lie in
             // [b0[m], b1[m]]).  A slope below 1e-30 leaves the range as is;
             // otherwise 1/k < ~1e30 and the estimates are finite.
             if (fabsf(r.ka) >= 1e-30f) {
                 float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - SLACK);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + SLACK);
             }
             if (fabsf(r.kb) >= 1e-30f) {
                 float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - 1);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + 1);
+                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - SLACK);
+                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + SLACK);
             }
             if (t0 >= t1) continue;
             const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
@@ -1101,8 +1106,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
                     int E = A0 + B0 + row;
                     bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
-#pragma unroll 2
-                    for (int t = t0; t < t1; ++t) {
+                    // the loop ends on the slab coordinate itself (exact small
+                    // integers in FP32), two slabs per trip
+                    const float tend = (float)(t1 - r.tc);
+                    auto slab = [&]() {
                         tw += 1.0f;
                         const float a1 = fmaf(r.ka, tw, r.Ac), bb1 = fmaf(r.kb, tw, r.Bc);
                         int ca1, cb1;
@@ -1155,6 +1162,12 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                         E = X + St;
                         ia0 = ia1;
                         ib0 = ib1;
+                    };
+                    for (;;) {
+                        slab();
+                        if (tw >= tend) break;
+                        slab();
+                        if (tw >= tend) break;
                     }
                 };
                 if (near_unit_slope(r))
