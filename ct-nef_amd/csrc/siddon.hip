@@ -894,6 +894,12 @@ __device__ __forceinline__ void lds_deposit_q(char* base, int byte_off, int q, i
     atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), v);
 }
 
+// Fixed-point deposit of any quantum q (sign-extended to 64 bits).
+__device__ __forceinline__ void lds_deposit_s(char* base, int byte_off, int q)
+{
+    atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), (unsigned long long)(long long)q);
+}
+
 __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
                                               const int hi[3], bool tight, int& ia0, int& ia1, int& ib0,
                                               int& ib1)
@@ -1098,8 +1104,9 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
                 const int qs = FIX ? (int)sv : 0;               // |sv| <= 2^30
                 const int hi = (FIX && sv < 0.0f) ? -1 : 0;
-                auto walk = [&](auto step_tag) {
+                auto walk = [&](auto step_tag, auto nonneg_tag) {
                     constexpr bool STEP = decltype(step_tag)::value;
+                    constexpr bool NONNEG = decltype(nonneg_tag)::value;
                     float tw = tf;
                     int ca0 = cell_floor(a0), cb0 = cell_floor(bb0);
                     float fra0 = cell_frac(a0), frb0 = cell_frac(bb0);
@@ -1140,13 +1147,25 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                             // the three pieces as differences of the truncated
                             // prefix sums f1 sv, f2 sv, sv: they add up to the
                             // slab's whole contribution exactly, and all share
-                            // the ray's sign, so zeros are skipped and the high
-                            // word is the ray's constant
+                            // the ray's sign.  The last piece is nonzero only
+                            // when the slab crosses both transverse boundaries:
+                            // it is skipped when zero and its high word is the
+                            // ray's constant.  The first and middle pieces are
+                            // added even when zero (cheaper than the test): in
+                            // a wavefront of non-negative rays with the high
+                            // word 0, otherwise sign-extended (a zero quantum
+                            // of a negative ray must not carry the sign word).
                             const int q1 = (int)(f1 * sv), q2 = (int)(f2 * sv);
                             const int d2 = q2 - q1, d3 = qs - q2;
-                            if (ia0 && ib0 && q1 != 0) lds_deposit_q(ab, E, q1, hi);
-                            if (i2 && d2 != 0) lds_deposit_q(ab, E + (afirst ? dA : dB), d2, hi);
-                            if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, hi);
+                            if constexpr (NONNEG) {
+                                if (ia0 && ib0) lds_deposit_q(ab, E, q1, 0);
+                                if (i2) lds_deposit_q(ab, E + (afirst ? dA : dB), d2, 0);
+                                if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, 0);
+                            } else {
+                                if (ia0 && ib0) lds_deposit_s(ab, E, q1);
+                                if (i2) lds_deposit_s(ab, E + (afirst ? dA : dB), d2);
+                                if (ia1 && ib1 && d3 != 0) lds_deposit_q(ab, X, d3, hi);
+                            }
                         } else {
                             const float w1 = f1, w2 = f2 - f1, w3 = 1.0f - f2;
                             if (ia0 && ib0 && w1 > 0.0f) lds_deposit<FIX>(ab, E, w1 * sv);
@@ -1170,10 +1189,17 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                         if (tw >= tend) break;
                     }
                 };
-                if (near_unit_slope(r))
-                    walk(std::true_type{});
+                auto walk_slope = [&](auto nonneg_tag) {
+                    if (near_unit_slope(r))
+                        walk(std::true_type{}, nonneg_tag);
+                    else
+                        walk(std::false_type{}, nonneg_tag);
+                };
+                // wave-uniform: every walking ray of the wavefront non-negative
+                if (FIX && __all(hi == 0))
+                    walk_slope(std::true_type{});
                 else
-                    walk(std::false_type{});
+                    walk_slope(std::false_type{});
                 continue;
             }
             for (int t = t0; t < t1; ++t) {
