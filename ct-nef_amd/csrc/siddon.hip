@@ -900,16 +900,16 @@ __device__ __forceinline__ void lds_deposit_s(char* base, int byte_off, int q)
     atomicAdd(reinterpret_cast<unsigned long long*>(base + byte_off), (unsigned long long)(long long)q);
 }
 
-__device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
-                                              const int hi[3], bool tight, int& ia0, int& ia1, int& ib0,
-                                              int& ib1)
+// Detector cells whose rays can meet the voxel box [lo, hi) at view (c, s),
+// as (ia0, ia1, ib0, ib1) (empty when ia1 < ia0).  Computed by 8 lanes
+// together: lane q (0..7) of the group takes corner q.
+__device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
+                                              const int hi[3], bool tight, int q)
 {
     // Fractional detector indices of the 8 corners (flat: a = t SDD / U,
     // b = z SDD / U; cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)),
-    // with the margin chosen below.  Lane l computes corner l & 7 and the groups of
-    // 8 lanes reduce by butterfly, so every lane ends with the same bounds;
-    // readfirstlane makes them scalar.
-    const int q = threadIdx.x & 7;
+    // with the margin chosen below.  The 8 lanes reduce by butterfly, so
+    // every lane of the group ends with the same bounds.
     const double x = V.lo[0] + V.vs[0] * ((q & 1) ? hi[0] : lo[0]);
     const double y = V.lo[1] + V.vs[1] * ((q & 2) ? hi[1] : lo[1]);
     const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]);
@@ -936,10 +936,8 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
         fb_mx = fmaxf(fb_mx, __shfl_xor(fb_mx, m));
         behind |= __shfl_xor(behind, m);
     }
-    if (__builtin_amdgcn_readfirstlane(behind)) {  // block reaches behind the source: take the whole detector
-        ia0 = 0; ia1 = G.na - 1; ib0 = 0; ib1 = G.nb - 1;
-        return;
-    }
+    if (behind)   // block reaches behind the source: take the whole detector
+        return make_int4(0, G.na - 1, 0, G.nb - 1);
     if (tight && !G.cyl) {
         // Flat detector, exact rule: the block's projection is the convex
         // hull of its corners' (a linear-fractional map), and cell i's ray
@@ -948,25 +946,26 @@ __device__ __forceinline__ void shadow_of_box(const VolDev& V, const DetDev& G, 
         // the FP32 rounding of these indices and of the walk's coordinates
         // many times over.
         constexpr float M = 0.125f;
-        ia0 = __builtin_amdgcn_readfirstlane(max(0, ceil_clamp(fa_mn - M, -2, G.na + 1)));
-        ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, floor_clamp(fa_mx + M, -2, G.na + 1)));
-        ib0 = __builtin_amdgcn_readfirstlane(max(0, ceil_clamp(fb_mn - M, -2, G.nb + 1)));
-        ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, floor_clamp(fb_mx + M, -2, G.nb + 1)));
-        return;
+        return make_int4(max(0, ceil_clamp(fa_mn - M, -2, G.na + 1)), min(G.na - 1, floor_clamp(fa_mx + M, -2, G.na + 1)),
+                         max(0, ceil_clamp(fb_mn - M, -2, G.nb + 1)), min(G.nb - 1, floor_clamp(fb_mx + M, -2, G.nb + 1)));
     }
     // one cell of margin: on a cylindrical detector the row coordinate of an
     // edge point can exceed its corners'; the reference rule can credit a
     // cell next to its exact segment
-    ia0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fa_mn, -2, G.na + 1) - 1));
-    ia1 = __builtin_amdgcn_readfirstlane(min(G.na - 1, ceil_clamp(fa_mx, -2, G.na + 1) + 1));
-    ib0 = __builtin_amdgcn_readfirstlane(max(0, floor_clamp(fb_mn, -2, G.nb + 1) - 1));
-    ib1 = __builtin_amdgcn_readfirstlane(min(G.nb - 1, ceil_clamp(fb_mx, -2, G.nb + 1) + 1));
+    return make_int4(max(0, floor_clamp(fa_mn, -2, G.na + 1) - 1), min(G.na - 1, ceil_clamp(fa_mx, -2, G.na + 1) + 1),
+                     max(0, floor_clamp(fb_mn, -2, G.nb + 1) - 1), min(G.nb - 1, ceil_clamp(fb_mx, -2, G.nb + 1) + 1));
 }
+
+// A block's shadows are computed SHV views at a time into an LDS table by all
+// of the workgroup's lanes (8 per view), instead of by every wavefront for
+// every view.
+constexpr int SHV = 1024;
+
 
 template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
                                                int bx, int by, int bz, TileAcc<TX, TY, TZ, acc_t<FIX>>& acc,
-                                               float S, bool accumulate)
+                                               int4* __restrict__ shadows, float S, bool accumulate)
 {
     using T = acc_t<FIX>;
     constexpr int SX = TileAcc<TX, TY, TZ, T>::SX;
@@ -982,10 +981,22 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
     __syncthreads();
     for (int iv = 0; iv < P.nv; ++iv) {
-        const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
-        int ia0, ia1, ib0, ib1;
-        shadow_of_box(P.V, P.G, c, s, b0, b1, !REF, ia0, ia1, ib0, ib1);
+        const int slot = iv % SHV;
+        if (slot == 0) {
+            __syncthreads();   // every wavefront is done with the previous views' entries
+            const int n = min(SHV, P.nv - iv);
+            for (int e = tid >> 3; e < n; e += 8 * NW) {   // the 8 lanes of a group share e
+                const int jv = iv + e;
+                const int4 sh = shadow_of_box(P.V, P.G, P.trig[2 * jv], P.trig[2 * jv + 1], b0, b1, !REF, tid & 7);
+                if ((tid & 7) == 0) shadows[e] = sh;
+            }
+            __syncthreads();
+        }
+        const int4 sh = shadows[slot];
+        const int ia0 = __builtin_amdgcn_readfirstlane(sh.x), ia1 = __builtin_amdgcn_readfirstlane(sh.y);
+        const int ib0 = __builtin_amdgcn_readfirstlane(sh.z), ib1 = __builtin_amdgcn_readfirstlane(sh.w);
         if (ia1 < ia0 || ib1 < ib0) continue;
+        const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
         // The shadow's W x H detector cells are numbered column by column
         // and dealt out densely: wavefront w takes cells w*64 + lane +
         // j*64*NW, so no lane idles for rounding the shadow up to whole
@@ -1255,8 +1266,10 @@ __global__ __launch_bounds__(64 * NW, 4) void k_siddon_adj_tile(Launch P, const 
                                                                 const unsigned* __restrict__ maxbits, float sbound)
 {
     __shared__ TileAcc<TX, TY, TZ, acc_t<FIX>> acc;
+    __shared__ int4 shadows[SHV];
     const float S = FIX ? fixed_scale(*maxbits, sbound) : 1.0f;
-    adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, S, false);
+    adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, shadows, S,
+                                                 false);
 }
 
 // Launch of one tile-adjoint variant.
@@ -1722,6 +1735,7 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
                                                         long long total, char* ws_base, unsigned flags)
 {
     __shared__ TileAcc<BTX, BTY, BTZ, acc_t<true>> acc;
+    __shared__ int4 shadows[SHV];
     const long long t = (long long)blockIdx.x + 65536ll * (long long)blockIdx.y;
     if (t >= total) return;
     const int p = __builtin_amdgcn_readfirstlane(find_problem(probs, nprob, t, true));
@@ -1735,7 +1749,7 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
     const Launch& L = batch_launches(ws_base)[p];
     const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
     adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
-                                                     S, false);
+                                                     shadows, S, false);
 }
 
 }  // namespace
