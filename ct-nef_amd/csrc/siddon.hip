@@ -380,8 +380,8 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
     int ca0 = cell_floor(a0), cb0 = cell_floor(b0);           // exact rule
     float fra0 = cell_frac(a0), frb0 = cell_frac(b0);
     const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
-#pragma unroll 2
-    for (int t = tmin; t < tmax; ++t) {
+    auto slab = [&](int t, auto ranged_tag) {
+        constexpr bool RANGED = decltype(ranged_tag)::value;
         tf += 1.0f;
         const float a1 = fmaf(r.ka, tf, r.Ac), b1 = fmaf(r.kb, tf, r.Bc);
         Seg s;
@@ -400,7 +400,7 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
             frb1 = cell_frac(b1);
             slab_exact<MODE == 2>(ca0, ca1, fra0, cb0, cb1, frb0, r, pia, pib, s);
         }
-        const bool act = mine && (t >= r.s0) && (t < r.s1) && s.ok;
+        const bool act = mine && s.ok && (!RANGED || ((t >= r.s0) && (t < r.s1)));
         const int o1 = (s.a1 + __mul24(s.b1, w.sb) + tb) * 4;
         const int o2 = (s.a2 + __mul24(s.b2, w.sb) + tb) * 4;
         const int o3 = (s.a3 + __mul24(s.b3, w.sb) + tb) * 4;
@@ -432,6 +432,22 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
             ca0 = ca1; cb0 = cb1; fra0 = fra1; frb0 = frb1;
         }
         tb += w.st;
+    };
+    if (MODE == 0) {
+        // the forward splits the wavefront's slab range: in [m0, m1) every
+        // walking ray is inside its own range [s0, s1), so only the ends test it
+        const int m0 = __builtin_amdgcn_readfirstlane(wave_max_i32(mine ? r.s0 : tmin));
+        const int m1 = __builtin_amdgcn_readfirstlane(wave_min_i32(mine ? r.s1 : tmax));
+        const int ta = min(max(m0, tmin), tmax), tb2 = max(min(m1, tmax), ta);
+#pragma unroll 2
+        for (int t = tmin; t < ta; ++t) slab(t, std::true_type{});
+#pragma unroll 2
+        for (int t = ta; t < tb2; ++t) slab(t, std::false_type{});
+#pragma unroll 2
+        for (int t = tb2; t < tmax; ++t) slab(t, std::true_type{});
+    } else {
+#pragma unroll 2
+        for (int t = tmin; t < tmax; ++t) slab(t, std::true_type{});
     }
     return acc;
 }
