@@ -978,6 +978,21 @@ __device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, 
 constexpr int SHV = 1024;
 
 
+// What a ray walking axis m needs of its block: the slab range on m, the
+// block's first centred cell and extent on the transverse axes a and b, the
+// MARGIN-widened box in centred coordinates, and the LDS byte strides (one
+// record per m, read by each lane for its ray's axis; selecting them from the
+// block bounds per lane costs ~40 instructions per ray).
+struct BlockAxis {
+    int b0m, b1m;
+    int la, lb;
+    unsigned ea, eb;
+    float alo, ahi, blo, bhi;
+    int Sa, Sb, St;
+    int row0;        // LDS byte offset of slab 0's centred cell (0, 0): row = t St + row0
+    int pad_[2];
+};
+
 template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
                                                int bx, int by, int bz, TileAcc<TX, TY, TZ, acc_t<FIX>>& acc,
@@ -995,6 +1010,28 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     const int b0[3] = {bx * TX, by * TY, bz * TZ};
     const int ext[3] = {min(TX, nx - b0[0]), min(TY, ny - b0[1]), min(TZ, nz - b0[2])};
     const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
+    constexpr float MARGIN = 1.0f / 64.0f;   // see the range tightening below
+    __shared__ BlockAxis bax[3];
+    if (tid < 3) {
+        const int m = tid, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
+        BlockAxis X;
+        X.b0m = b0[m];
+        X.b1m = b1[m];
+        X.la = b0[a] - (P.V.n[a] >> 1);
+        X.lb = b0[b] - (P.V.n[b] >> 1);
+        X.ea = (unsigned)ext[a];
+        X.eb = (unsigned)ext[b];
+        X.alo = (float)X.la - MARGIN;
+        X.ahi = (float)(X.la + ext[a]) + MARGIN;
+        X.blo = (float)X.lb - MARGIN;
+        X.bhi = (float)(X.lb + ext[b]) + MARGIN;
+        X.Sa = ES * ((m == 0) ? SX : 1);
+        X.Sb = ES * ((m == 2) ? SX : SXY);
+        X.St = ES * ((m == 0) ? 1 : ((m == 1) ? SX : SXY));
+        X.row0 = -X.b0m * X.St - X.la * X.Sa - X.lb * X.Sb;
+        X.pad_[0] = X.pad_[1] = 0;
+        bax[m] = X;
+    }
     __syncthreads();
     for (int iv = 0; iv < P.nv; ++iv) {
         const int slot = iv % SHV;
@@ -1060,9 +1097,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             Ray r;
             setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
             if (r.dom < 0) continue;
-            const int m = r.dom, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
+            const int m = r.dom;
+            const BlockAxis& X = bax[m];
             // slabs of the block on the walk axis, within the ray's own range
-            int t0 = max(r.s0, b0[m]), t1 = min(r.s1, b1[m]);
+            int t0 = max(r.s0, X.b0m), t1 = min(r.s1, X.b1m);
             if (t0 >= t1) continue;
             // FP32 coordinates of the ray at the block's slab faces decide the
             // transverse overlap, as the walk will compute them.  The box is
@@ -1077,13 +1115,11 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             // touches a widened face.  The reference rule can credit a
             // neighbour of its exact cell, so it keeps one slab of slack on
             // each side.
-            constexpr float MARGIN = 1.0f / 64.0f;
             constexpr int SLACK = REF ? 1 : 0;
             const float fa0 = fmaf(r.ka, (float)(t0 - r.tc), r.Ac), fa1 = fmaf(r.ka, (float)(t1 - r.tc), r.Ac);
             const float fb0 = fmaf(r.kb, (float)(t0 - r.tc), r.Bc), fb1 = fmaf(r.kb, (float)(t1 - r.tc), r.Bc);
-            const int ha = P.V.n[a] >> 1, hb = P.V.n[b] >> 1;
-            const float alo = (float)(b0[a] - ha) - MARGIN, ahi = (float)(b1[a] - ha) + MARGIN;
-            const float blo = (float)(b0[b] - hb) - MARGIN, bhi = (float)(b1[b] - hb) + MARGIN;
+            const float alo = X.alo, ahi = X.ahi, blo = X.blo, bhi = X.bhi;
+            const int bm0 = X.b0m, bm1 = X.b1m;
             if (fmaxf(fa0, fa1) < alo || fminf(fa0, fa1) > ahi || fmaxf(fb0, fb1) < blo || fminf(fb0, fb1) > bhi)
                 continue;
             // tighten [t0, t1) to the slabs where the ray is within the margin
@@ -1094,14 +1130,14 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             if (fabsf(r.ka) >= 1e-30f) {
                 float ta = (alo - r.Ac) * r.inv_ka + (float)r.tc, tb = (ahi - r.Ac) * r.inv_ka + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - SLACK);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + SLACK);
+                t0 = max(t0, floor_clamp(ta, bm0, bm1 + 2) - SLACK);
+                t1 = min(t1, ceil_clamp(tb, bm0 - 2, bm1) + SLACK);
             }
             if (fabsf(r.kb) >= 1e-30f) {
                 float ta = (blo - r.Bc) * r.inv_kb + (float)r.tc, tb = (bhi - r.Bc) * r.inv_kb + (float)r.tc;
                 if (ta > tb) { const float q = ta; ta = tb; tb = q; }
-                t0 = max(t0, floor_clamp(ta, b0[m], b1[m] + 2) - SLACK);
-                t1 = min(t1, ceil_clamp(tb, b0[m] - 2, b1[m]) + SLACK);
+                t0 = max(t0, floor_clamp(ta, bm0, bm1 + 2) - SLACK);
+                t1 = min(t1, ceil_clamp(tb, bm0 - 2, bm1) + SLACK);
             }
             if (t0 >= t1) continue;
             const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
@@ -1110,13 +1146,11 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             // (ca Sa + cb Sb + t St + base) * ES, with the strides of the walk
             // axis; [t0, t1) lies inside the block, so only ca and cb are
             // range-checked
-            const int Sa = ES * ((m == 0) ? SX : 1);
-            const int Sb = ES * ((m == 2) ? SX : SXY);
-            const int St = ES * ((m == 0) ? 1 : ((m == 1) ? SX : SXY));
-            const int la = b0[a] - ha, lb = b0[b] - hb;     // centred cell of the block's first a / b index
-            const unsigned ea = (unsigned)ext[a], eb = (unsigned)ext[b];
+            const int Sa = X.Sa, Sb = X.Sb, St = X.St;
+            const int la = X.la, lb = X.lb;     // centred cell of the block's first a / b index
+            const unsigned ea = X.ea, eb = X.eb;
             char* const ab = reinterpret_cast<char*>(af);
-            int row = (t0 - b0[m]) * St - la * Sa - lb * Sb;
+            int row = t0 * St + X.row0;
             float tf = (float)(t0 - r.tc);
             float a0 = fmaf(r.ka, tf, r.Ac), bb0 = fmaf(r.kb, tf, r.Bc);
             float fla0 = floorf(a0), flb0 = floorf(bb0);
