@@ -175,17 +175,32 @@ __device__ __forceinline__ double rcp64(double x)
     return fma(r, e, r);
 }
 
-// FP64 ray set-up (three reciprocals; the length per slab in FP32).
-// 'ref' selects the reference's x/y-only dominance rule.
-__device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
-                          unsigned flags, bool adjoint, Ray& r)
+// What the set-up needs of walk axis m and its transverse axes a, b: the
+// volume's extents and voxel sizes and the view's source in voxel units.
+struct AxisGeom {
+    int nm, na, nb;
+    double vsm, vsa, vsb, ivsm;
+    double uSm, uSa, uSb;
+};
+
+// The view's source in voxel units (the S of ray_points).
+__device__ __forceinline__ void source_voxels(const VolDev& V, const DetDev& G, double c, double s, double uS[3])
 {
-    double S[3], D[3], uS[3], d[3];
+    const double S[3] = {-G.sid * c, -G.sid * s, 0.0};
+    for (int i = 0; i < 3; ++i) uS[i] = (S[i] - V.lo[i]) * V.inv_vs[i];
+}
+
+// FP64 ray set-up (three reciprocals; the length per slab in FP32).
+// 'ref' selects the reference's x/y-only dominance rule.  geom(m) supplies the
+// AxisGeom of the ray's walk axis (selected from V per lane, or read from a
+// table), with the same values either way.
+template <class Geom>
+__device__ __forceinline__ void setup_ray_g(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
+                                            unsigned flags, bool adjoint, Geom&& geom, Ray& r)
+{
+    double S[3], D[3], d[3];
     ray_points(G, c, s, ia, ib, S, D);
-    for (int i = 0; i < 3; ++i) {
-        uS[i] = (S[i] - V.lo[i]) * V.inv_vs[i];
-        d[i] = (D[i] - S[i]) * V.inv_vs[i];
-    }
+    for (int i = 0; i < 3; ++i) d[i] = (D[i] - S[i]) * V.inv_vs[i];
     r.dom = -1;
     r.s0 = r.s1 = 0;
     r.tc = 0;
@@ -206,27 +221,28 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     if (d[m] == 0.0) return;
     const int a = (m == 0) ? 1 : 0;
     const int b = (m == 2) ? 1 : 2;
+    const AxisGeom g = geom(m);
     const double inv_dm = rcp64(d[m]);
     const double ka = d[a] * inv_dm, kb = d[b] * inv_dm;
     const double ika = (d[a] != 0.0) ? d[m] * rcp64(d[a]) : 0.0;
     const double ikb = (d[b] != 0.0) ? d[m] * rcp64(d[b]) : 0.0;
-    const double Aa = uS[a] - ka * uS[m], Ab = uS[b] - kb * uS[m];
+    const double Aa = g.uSa - ka * g.uSm, Ab = g.uSb - kb * g.uSm;
     // The exact walk clips to the volume box.  The reference rule can credit an
     // in-range voxel from a slab whose exact segment lies just outside the box
     // (the Q2 attribution at a corner), so reference mode clips to the box
     // widened by one cell on each transverse side; its per-voxel range tests
     // then decide exactly as siddon.cu.cc:80-176 does.
     const double mg = ref ? 1.0 : 0.0;
-    double tin = 0.0, tout = (double)V.n[m];
-    clip_axis(ka, ika, Aa, -mg, V.n[a] + mg, tin, tout);
-    clip_axis(kb, ikb, Ab, -mg, V.n[b] + mg, tin, tout);
+    double tin = 0.0, tout = (double)g.nm;
+    clip_axis(ka, ika, Aa, -mg, g.na + mg, tin, tout);
+    clip_axis(kb, ikb, Ab, -mg, g.nb + mg, tin, tout);
     if (!(tout > tin)) return;
     int s0 = (int)floor(tin), s1 = (int)ceil(tout);
     s0 = max(s0, 0);
-    s1 = min(s1, V.n[m]);
+    s1 = min(s1, g.nm);
     if (s1 <= s0) return;
     const int tc = (s0 + s1) >> 1;
-    const int ha = V.n[a] >> 1, hb = V.n[b] >> 1;
+    const int ha = g.na >> 1, hb = g.nb >> 1;
     r.dom = m;
     r.s0 = s0;
     r.s1 = s1;
@@ -239,10 +255,36 @@ __device__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, 
     r.inv_kb = (float)ikb;
     r.pa = ka > 0.0 ? 1.0f : 0.0f;
     r.pb = kb > 0.0 ? 1.0f : 0.0f;
-    const double qa = ka * V.vs[a] * V.inv_vs[m], qb = kb * V.vs[b] * V.inv_vs[m];
-    double sc = V.vs[m] * (double)sqrtf((float)(1.0 + qa * qa + qb * qb));   // rounded to FP32 below
+    const double qa = ka * g.vsa * g.ivsm, qb = kb * g.vsb * g.ivsm;
+    double sc = g.vsm * (double)sqrtf((float)(1.0 + qa * qa + qb * qb));   // rounded to FP32 below
     if (flags & NEFCT_F_INV_L2) sc = sc / (xd * xd + yd * yd + zd * zd);
     r.scale = (float)sc;
+}
+
+// AxisGeom of walk axis m from the volume record.
+__device__ __forceinline__ AxisGeom axis_geom(const VolDev& V, const double uS[3], int m)
+{
+    const int a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
+    AxisGeom g;
+    g.nm = V.n[m];
+    g.na = V.n[a];
+    g.nb = V.n[b];
+    g.vsm = V.vs[m];
+    g.vsa = V.vs[a];
+    g.vsb = V.vs[b];
+    g.ivsm = V.inv_vs[m];
+    g.uSm = uS[m];
+    g.uSa = uS[a];
+    g.uSb = uS[b];
+    return g;
+}
+
+__device__ __forceinline__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
+                                          unsigned flags, bool adjoint, Ray& r)
+{
+    double uS[3];
+    source_voxels(V, G, c, s, uS);
+    setup_ray_g(V, G, c, s, ia, ib, flags, adjoint, [&](int m) { return axis_geom(V, uS, m); }, r);
 }
 
 __device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts& Ly)
@@ -972,10 +1014,10 @@ __device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, 
                      max(0, floor_clamp(fb_mn, -2, G.nb + 1) - 1), min(G.nb - 1, ceil_clamp(fb_mx, -2, G.nb + 1) + 1));
 }
 
-// A block's shadows are computed SHV views at a time into an LDS table by all
-// of the workgroup's lanes (8 per view), instead of by every wavefront for
-// every view.
-constexpr int SHV = 1024;
+// A block's shadows (and the views' sources in voxel units) are computed SHV
+// views at a time into LDS tables by all of the workgroup's lanes (8 per
+// view), instead of by every wavefront for every view.
+constexpr int SHV = 512;
 
 
 // What a ray walking axis m needs of its block: the slab range on m, the
@@ -1012,6 +1054,12 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     const int b1[3] = {b0[0] + ext[0], b0[1] + ext[1], b0[2] + ext[2]};
     constexpr float MARGIN = 1.0f / 64.0f;   // see the range tightening below
     __shared__ BlockAxis bax[3];
+    __shared__ AxisGeom vax[3];          // volume part of each walk axis's AxisGeom
+    __shared__ double src_tab[SHV][3];   // source in voxel units per view of the chunk
+    if (tid < 3) {
+        double zero[3] = {0.0, 0.0, 0.0};
+        vax[tid] = axis_geom(P.V, zero, tid);
+    }
     if (tid < 3) {
         const int m = tid, a = (m == 0) ? 1 : 0, b = (m == 2) ? 1 : 2;
         BlockAxis X;
@@ -1040,8 +1088,12 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             const int n = min(SHV, P.nv - iv);
             for (int e = tid >> 3; e < n; e += 8 * NW) {   // the 8 lanes of a group share e
                 const int jv = iv + e;
-                const int4 sh = shadow_of_box(P.V, P.G, P.trig[2 * jv], P.trig[2 * jv + 1], b0, b1, !REF, tid & 7);
-                if ((tid & 7) == 0) shadows[e] = sh;
+                const double cj = P.trig[2 * jv], sj = P.trig[2 * jv + 1];
+                const int4 sh = shadow_of_box(P.V, P.G, cj, sj, b0, b1, !REF, tid & 7);
+                if ((tid & 7) == 0) {
+                    shadows[e] = sh;
+                    source_voxels(P.V, P.G, cj, sj, src_tab[e]);
+                }
             }
             __syncthreads();
         }
@@ -1095,7 +1147,17 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             if (ok_n) pv_n = pv[view_base + (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n];
             if (!ok) continue;
             Ray r;
-            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r);
+            // the set-up's per-axis values from the LDS tables (the same
+            // values setup_ray selects from the volume record)
+            setup_ray_g(P.V, P.G, c, s, ia, ib, P.flags, true,
+                        [&](int m) {
+                            AxisGeom g = vax[m];
+                            g.uSm = src_tab[slot][m];
+                            g.uSa = src_tab[slot][(m == 0) ? 1 : 0];
+                            g.uSb = src_tab[slot][(m == 2) ? 1 : 2];
+                            return g;
+                        },
+                        r);
             if (r.dom < 0) continue;
             const int m = r.dom;
             const BlockAxis& X = bax[m];
