@@ -131,6 +131,88 @@ def batched_leg(dev, iters, log_every=True):
     }
 
 
+def _timed(fn, reps):
+    """Mean HIP-event time (s) of fn() over reps calls on torch's stream, after one untimed call."""
+    import torch
+    fn()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps, out
+
+
+def fdk_leg(wl, img, sino, ang, reps=2):
+    """FDK of the workload's projections: the short scan (180 degrees + fan
+    angle) of the first views, cosine and Parker weights, ram-lak row filter,
+    voxel-driven backprojection; each stage timed on its own."""
+    import torch
+    from nefct_hip import fdk as F
+    vol, det = wl.vol, wl.det
+    nv_full = ang.numel()
+    dang = 2.0 * math.pi / nv_full
+    gamma = math.atan(0.5 * det.na * det.da / det.SAD)
+    nv = min(nv_full, int(math.ceil((math.pi + 2.0 * gamma) / dang)) + 1)
+    p = sino[:nv].contiguous()
+    a = ang[:nv].contiguous()
+    scale = F.fdk_scale(det, nv, dang, True)
+    log(f"[bench] FDK leg: short scan of {nv} views")
+    t_w, q = _timed(lambda: F.preweight(p, det, parker=True, dang_rad=dang, direction=1, scale=scale), reps)
+    t_f, f = _timed(lambda: F.ramp_filter(q, "ram-lak"), reps)
+    t_b, rec = _timed(lambda: F.backproject(f, vol, det, a), reps)
+    nvox = vol.grid[0] * vol.grid[1] * vol.grid[2]
+    # inner cylinder (radius 0.4 of the width), central half of the slices: no truncation or
+    # cone-beam edge effects
+    nz, ny, nx = vol.shape_zyx
+    yy, xx = torch.meshgrid(torch.arange(ny, device=img.device), torch.arange(nx, device=img.device), indexing="ij")
+    inner = ((xx - (nx - 1) / 2) ** 2 + (yy - (ny - 1) / 2) ** 2) < (0.4 * nx) ** 2
+    zs = slice(nz // 4, 3 * nz // 4)
+    d = (rec[zs][:, inner] - img[zs][:, inner]).double()
+    rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(img[zs][:, inner].double()))
+    proj_bytes = 4 * p.numel()
+    bp_bytes = proj_bytes + 4 * nvox + 8 * nv
+    log(f"[bench] FDK leg: weight {t_w * 1e3:.2f} ms, filter {t_f * 1e3:.2f} ms, backprojection {t_b * 1e3:.2f} ms")
+    return {
+        "views": nv,
+        "ms": round((t_w + t_f + t_b) * 1e3, 3),
+        "stage_ms": {"preweight": round(t_w * 1e3, 3), "filter": round(t_f * 1e3, 3),
+                     "backproject": round(t_b * 1e3, 3)},
+        "backproject_voxel_views_per_s": nvox * nv / t_b,
+        "backproject_hbm": {"algorithmic_bytes": bp_bytes, "achieved_GBs": round(bp_bytes / t_b / 1e9, 1),
+                            "frac": round(bp_bytes / t_b / 1e9 / HBM_PEAK_GBS, 4)},
+        "filter_gflops": round(2.0 * p.numel() * det.na / t_f / 1e9, 1),
+        "preweight_hbm": {"algorithmic_bytes": 2 * proj_bytes, "achieved_GBs": round(2 * proj_bytes / t_w / 1e9, 1),
+                          "frac": round(2 * proj_bytes / t_w / 1e9 / HBM_PEAK_GBS, 4)},
+        "rel_err_vs_phantom_inner": round(rel, 4),
+    }
+
+
+def warp_leg(wl, img, reps=3):
+    """Trilinear warp of the workload's volume by a smooth displacement field
+    (forward, transpose scatter, displacement gradient), HBM rate of each."""
+    import torch
+    from nefct_hip.deform import deform_adjoint_op, deform_dvf_grad_op, deform_op
+    from nefct_hip.phantom import gaussian_dvf
+    nvox = img.numel()
+    dvf = gaussian_dvf(wl.vol.shape_zyx, 7, voxel_mm=wl.vol.voxel[0], device=img.device).float().contiguous()
+    g = torch.ones_like(img)
+    t_f, _ = _timed(lambda: deform_op(img, dvf[0], dvf[1], dvf[2], 1), reps)
+    t_a, _ = _timed(lambda: deform_adjoint_op(g, dvf[0], dvf[1], dvf[2], 1), reps)
+    t_g, _ = _timed(lambda: deform_dvf_grad_op(img, g, dvf[0], dvf[1], dvf[2], 1), reps)
+    # algorithmic bytes per voxel: forward reads the image and 3 field components, writes 1;
+    # the transpose reads g and the field, writes the image gradient; the field gradient reads
+    # the image, g and the field, writes 3 components
+    b_f, b_a, b_g = 20 * nvox, 20 * nvox, 32 * nvox
+    log(f"[bench] warp leg: forward {t_f * 1e3:.2f} ms, transpose {t_a * 1e3:.2f} ms, field gradient {t_g * 1e3:.2f} ms")
+    rate = lambda b, t: {"ms": round(t * 1e3, 3), "algorithmic_bytes": b, "achieved_GBs": round(b / t / 1e9, 1),
+                         "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4)}
+    return {"volume": list(wl.vol.grid), "forward": rate(b_f, t_f), "transpose": rate(b_a, t_a),
+            "field_gradient": rate(b_g, t_g)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +229,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain leg")
     ap.add_argument("--batched-iters", type=int, default=2)
+    ap.add_argument("--no-fdk", action="store_true", help="skip the FDK and warp legs")
     ap.add_argument("--calib", action="store_true",
                     help="profiling only: first copy 256 MiB device-to-device (a known-size dispatch that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -227,6 +310,14 @@ def main():
     b_dom = bytes_adj if t_adj > t_fwd else bytes_fwd
     achieved = b_dom / t_dom / 1e9
 
+    fdk = warp_rates = None
+    if not args.no_fdk and not args.views and args.only == "pair":
+        sino_fdk = nh.project(img, vol, det, ang)
+        fdk = fdk_leg(wl, img, sino_fdk, ang)
+        del sino_fdk
+        warp_rates = warp_leg(wl, img)
+        torch.cuda.empty_cache()
+
     batched = None
     if not args.no_batched and not args.views and args.only == "pair":
         del sino_fixed
@@ -287,6 +378,8 @@ def main():
             "avg_launch_ms": round(t_dom * 1e3, 3),
         },
         "batched": batched,
+        "fdk": fdk,
+        "warp": warp_rates,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -41,6 +41,17 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
+// 1/x in FP64 from the hardware reciprocal and two Newton steps (a few ulp at
+// most, against ~10 instructions for a correctly rounded division).
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 #define NEFCT_CHECK_LAUNCH()                                        \
     do {                                                            \
         hipError_t e_ = hipGetLastError();                          \

@@ -20,17 +20,21 @@
 namespace {
 
 // q[iv][ib][ia] = p * SDD / sqrt(SDD^2 + a^2 + b^2) * parker(iv, ia) * scale
-__global__ void k_fdk_preweight(const float* __restrict__ p, float* __restrict__ q, int nv, int na, int nb,
-                                double SAD, double da, double ai, double db, double bi, int parker, double dang_deg,
-                                int direction, double scale)
+// Thread = (column ia, view iv) over a run of PW_ROWS detector rows: the
+// Parker factor (a function of the view and the column) is computed once.
+constexpr int PW_ROWS = 64;
+
+__global__ __launch_bounds__(256) void k_fdk_preweight(const float* __restrict__ p, float* __restrict__ q, int nv,
+                                                       int na, int nb, double SAD, double da, double ai, double db,
+                                                       double bi, int parker, double dang_deg, int direction,
+                                                       double scale)
 {
-    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ib = blockIdx.y;
-    const int iv = blockIdx.z;
+    const int ia = blockIdx.x * 256 + threadIdx.x;
+    const int iv = blockIdx.y;
+    const int ib0 = blockIdx.z * PW_ROWS, ib1 = min(nb, ib0 + PW_ROWS);
     if (ia >= na) return;
     const double a = (ai + ia) * da - na * da * 0.5;
-    const double b = (bi + ib) * db - nb * db * 0.5;
-    double w = SAD / sqrt(SAD * SAD + a * a + b * b) * scale;
+    double pk = 1.0;
     if (parker) {
         // fan angles of the first and last column and of this one, degrees
         const double rad2deg = 57.295779513082320876798;
@@ -41,7 +45,6 @@ __global__ void k_fdk_preweight(const float* __restrict__ p, float* __restrict__
         const double n180 = 180.0 / dang_deg;
         const double pw = ((n180 - nv) * 0.5 + (iv + 1)) / n180 * 180.0 + direction * gam;
         const double pi2 = 1.5707963267948966192313;
-        double pk = 1.0;
         if (pw <= fan * 0.5 && pw > -fan * 0.5) {
             const double sv = sin(pi2 * (fan * 0.5 + pw) / fan);
             pk = sv > 0.0 ? sv * sv : 0.0;
@@ -51,68 +54,115 @@ __global__ void k_fdk_preweight(const float* __restrict__ p, float* __restrict__
         } else if (pw < -fan * 0.5 || pw > 180.0 + fan * 0.5) {
             pk = 0.0;
         }
-        w *= pk;
     }
-    const size_t id = (size_t)ia + (size_t)na * ((size_t)ib + (size_t)nb * iv);
-    q[id] = (float)(p[id] * w);
+    const double sa = SAD * SAD + a * a;
+    const double f = SAD * scale * pk;
+    for (int ib = ib0; ib < ib1; ++ib) {
+        const double b = (bi + ib) * db - nb * db * 0.5;
+        const size_t id = (size_t)ia + (size_t)na * ((size_t)ib + (size_t)nb * iv);
+        q[id] = (float)(p[id] * (f / sqrt(sa + b * b)));
+    }
 }
 
 // Row convolution with a (2 na - 1)-tap kernel h centred at na - 1:
 //   out[u] = sum_v in[v] * h[u - v + na - 1]
-// One workgroup per FR rows; rows and taps staged in LDS; every thread
-// produces 4 neighbouring outputs of one row with a sliding tap window.
-constexpr int FR = 4;
-constexpr int FMAX = 2048;   // longest detector row supported
+// computed as a circular convolution of length N (the smallest power of two
+// >= 2 na - 1, at least 64): kernel offsets |u - v| <= na - 1 < N / 2, so
+// the first na outputs of the circular convolution are exactly the linear
+// ones (the reference filters the same way, FFT of a zero-padded row,
+// filtering.m:29-41).  Each workgroup transforms the taps once into an LDS
+// response, then takes two rows at a time as the real and imaginary parts of
+// one complex row (the taps are real, so the two filtered rows come back as
+// the real and imaginary parts): FFT, multiply, inverse FFT through
+// conj(FFT(conj(.))) / N.  FFTs are radix-2 Stockham (self-sorting) in LDS.
+constexpr int FMAX = 2048;   // longest detector row supported (N <= 4096)
 
-__global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in, float* __restrict__ out,
-                                                    const float* __restrict__ h, int rows, int na)
+template <int N>
+__device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
 {
-    __shared__ float srow[FR][FMAX];
-    __shared__ float sh[2 * FMAX];
-    const int r0 = blockIdx.x * FR;
+    constexpr int H2 = N / 2;
+    for (int Ns = 1; Ns < N; Ns <<= 1) {
+        for (int j = threadIdx.x; j < H2; j += 256) {
+            const int k = j & (Ns - 1);
+            const float2 v0 = buf[a][j], u1 = buf[a][j + H2];
+            const float2 w = tw[k * (H2 / Ns)];   // exp(-2 pi i k / (2 Ns))
+            const float2 v1 = make_float2(u1.x * w.x - u1.y * w.y, u1.x * w.y + u1.y * w.x);
+            const int d = ((j - k) << 1) + k;
+            buf[a ^ 1][d] = make_float2(v0.x + v1.x, v0.y + v1.y);
+            buf[a ^ 1][d + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
+        }
+        __syncthreads();
+        a ^= 1;
+    }
+    return a;
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in, float* __restrict__ out,
+                                                    const float* __restrict__ h, long long rows, int na)
+{
+    constexpr int N = 1 << LOGN, H2 = N / 2;
+    __shared__ float2 buf[2][N];
+    __shared__ float2 resp[N];
+    __shared__ float2 tw[H2];
     const int tid = threadIdx.x;
-    const int ntap = 2 * na - 1;
-    for (int i = tid; i < ntap; i += 256) sh[i] = h[i];
-    sh[ntap] = 0.0f;
-    for (int r = 0; r < FR; ++r) {
-        const int row = r0 + r;
-        for (int i = tid; i < na; i += 256) srow[r][i] = (row < rows) ? in[(size_t)row * na + i] : 0.0f;
+    for (int t = tid; t < H2; t += 256) {
+        float sn, cs;
+        sincospif(-2.0f * (float)t / (float)N, &sn, &cs);
+        tw[t] = make_float2(cs, sn);
+    }
+    // the taps laid out circularly: offset d at index d mod N
+    for (int i = tid; i < N; i += 256) {
+        float v = 0.0f;
+        if (i <= na - 1)
+            v = h[na - 1 + i];
+        else if (i >= N - (na - 1))
+            v = h[na - 1 + i - N];
+        buf[0][i] = make_float2(v, 0.0f);
     }
     __syncthreads();
-    const int nq = (na + 3) / 4;  // groups of 4 outputs per row
-    for (int job = tid; job < FR * nq; job += 256) {
-        const int r = job / nq, u0 = (job % nq) * 4;
-        if (r0 + r >= rows) continue;
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-        // tap index for output u and input v: u - v + na - 1; start at v = 0
-        int k = u0 + na - 1;
-        float h0 = sh[k], h1 = (k + 1 < ntap) ? sh[k + 1] : 0.0f, h2 = (k + 2 < ntap) ? sh[k + 2] : 0.0f,
-              h3 = (k + 3 < ntap) ? sh[k + 3] : 0.0f;
-        for (int v = 0; v < na; ++v) {
-            const float x = srow[r][v];
-            a0 = fmaf(x, h0, a0);
-            a1 = fmaf(x, h1, a1);
-            a2 = fmaf(x, h2, a2);
-            a3 = fmaf(x, h3, a3);
-            // next input: taps shift down by one
-            h3 = h2;
-            h2 = h1;
-            h1 = h0;
-            --k;
-            h0 = (k >= 0) ? sh[k] : 0.0f;
+    const int sr = fft_stockham<N>(buf, 0, tw);
+    for (int i = tid; i < N; i += 256) resp[i] = buf[sr][i];
+    __syncthreads();
+    const float inv_n = 1.0f / (float)N;
+    for (long long pr = blockIdx.x; 2 * pr < rows; pr += gridDim.x) {
+        const long long r1 = 2 * pr, r2 = r1 + 1;
+        const bool has2 = r2 < rows;
+        for (int i = tid; i < N; i += 256) {
+            float x1 = 0.0f, x2 = 0.0f;
+            if (i < na) {
+                x1 = in[r1 * na + i];
+                if (has2) x2 = in[r2 * na + i];
+            }
+            buf[0][i] = make_float2(x1, x2);
         }
-        const size_t base = (size_t)(r0 + r) * na;
-        if (u0 + 0 < na) out[base + u0 + 0] = a0;
-        if (u0 + 1 < na) out[base + u0 + 1] = a1;
-        if (u0 + 2 < na) out[base + u0 + 2] = a2;
-        if (u0 + 3 < na) out[base + u0 + 3] = a3;
+        __syncthreads();
+        const int s1 = fft_stockham<N>(buf, 0, tw);
+        for (int i = tid; i < N; i += 256) {   // conj(Z H)
+            const float2 z = buf[s1][i], hh = resp[i];
+            buf[s1][i] = make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+        }
+        __syncthreads();
+        const int s2 = fft_stockham<N>(buf, s1, tw);
+        for (int i = tid; i < na; i += 256) {
+            const float2 y = buf[s2][i];
+            out[r1 * na + i] = y.x * inv_n;
+            if (has2) out[r2 * na + i] = -y.y * inv_n;
+        }
+        __syncthreads();   // the next pair overwrites buf[0]
     }
 }
 
 // Voxel-driven weighted backprojection.  Thread = (ix, iy, z-chunk of ZC
-// voxels); all views accumulate in registers; one write per voxel.
-constexpr int ZC = 8;
+// voxels); all views accumulate in registers; one write per voxel.  Per
+// (x, y, view) the detector column, magnification and distance weight are
+// computed once in FP64 (reciprocals instead of divisions) and shared by the
+// ZC voxels of the column; the row coordinate is FP32 per voxel.  With
+// PAIR (na >= 2, so ia0 <= na - 2) the two taps of a detector row are read as
+// one 8-byte load (4-byte aligned; gfx950 loads need only dword alignment).
+constexpr int ZC = 16;
 
+template <bool PAIR>
 __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, const double* __restrict__ trig,
                                                 const float* __restrict__ offsets, int nv, int nx, int ny, int nz,
                                                 double x0, double y0, double z0, double vx, double vy, double vz,
@@ -124,40 +174,58 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
     const int izc = blockIdx.z * ZC;
     if (ix >= nx || iy >= ny) return;
     const double x = x0 + (ix + 0.5) * vx, y = y0 + (iy + 0.5) * vy;
-    float acc[ZC];
+    float acc[ZC], zf[ZC];
 #pragma unroll
-    for (int k = 0; k < ZC; ++k) acc[k] = 0.0f;
+    for (int k = 0; k < ZC; ++k) {
+        acc[k] = 0.0f;
+        zf[k] = (float)(z0 + (izc + k + 0.5) * vz);
+    }
+    const int kn = min(ZC, nz - izc);
     const double ha = na * da * 0.5, hb = nb * db * 0.5;
+    const double inv_da = 1.0 / da;
     const float inv_db = (float)(1.0 / db);
+    const float fb0 = (float)(hb / db - bi);
+    const float nb1 = (float)(nb - 1);
     for (int iv = 0; iv < nv; ++iv) {
+        // view-uniform values first, before the lanes diverge
+        const float* qv = q + (size_t)nb * na * iv;
+        const float off = offsets ? offsets[iv] : 0.0f;
         const double c = trig[2 * iv], s = trig[2 * iv + 1];
         const double U = SID + x * c + y * s;
         if (!(U > 0.0)) continue;
-        const double mag = SAD / U;
-        const double fa = ((-x * s + y * c) * mag + ha) / da - ai;
+        const double iU = rcp64(U);
+        const double mag = SAD * iU;
+        const double fa = ((-x * s + y * c) * mag + ha) * inv_da - ai;
         if (!(fa >= 0.0 && fa <= (double)(na - 1))) continue;
         const int ia0 = min((int)floor(fa), max(na - 2, 0));
         const float ta = (float)(fa - ia0);
-        const float w = (float)((SID / U) * (SID / U));
-        const float off = offsets ? offsets[iv] : 0.0f;
-        const float magf = (float)mag;
-        const float fb0 = (float)((hb) / db - bi);
-        const float* qv = q + (size_t)nb * na * iv;
+        const float wr = (float)(SID * iU);
+        const float w = wr * wr;
+        const float mb = (float)mag * inv_db;
         const int ia1 = min(ia0 + 1, na - 1);
 #pragma unroll
         for (int k = 0; k < ZC; ++k) {
-            const int iz = izc + k;
-            if (iz >= nz) break;
-            const float zr = (float)(z0 + (iz + 0.5) * vz) - off;
-            const float fb = fmaf(zr * magf, inv_db, fb0);
-            if (!(fb >= 0.0f && fb <= (float)(nb - 1))) continue;
+            const float fb = fmaf(zf[k] - off, mb, fb0);
+            if (!(k < kn && fb >= 0.0f && fb <= nb1)) continue;
             const int ib0 = min((int)floorf(fb), max(nb - 2, 0));
             const int ib1 = min(ib0 + 1, nb - 1);
             const float tb = fb - (float)ib0;
             const float* r0p = qv + (size_t)ib0 * na;
             const float* r1p = qv + (size_t)ib1 * na;
-            const float top = fmaf(ta, r0p[ia1] - r0p[ia0], r0p[ia0]);
-            const float bot = fmaf(ta, r1p[ia1] - r1p[ia0], r1p[ia0]);
+            float top, bot;
+            if (PAIR) {
+                // 32-bit byte offsets from the view's base (a view has < 2^29 cells)
+                const unsigned o0 = 4u * (unsigned)(ib0 * na + ia0), o1 = o0 + 4u * (unsigned)((ib1 - ib0) * na);
+                const char* qb = reinterpret_cast<const char*>(qv);
+                float2 p0, p1;
+                __builtin_memcpy(&p0, qb + o0, 8);
+                __builtin_memcpy(&p1, qb + o1, 8);
+                top = fmaf(ta, p0.y - p0.x, p0.x);
+                bot = fmaf(ta, p1.y - p1.x, p1.x);
+            } else {
+                top = fmaf(ta, r0p[ia1] - r0p[ia0], r0p[ia0]);
+                bot = fmaf(ta, r1p[ia1] - r1p[ia0], r1p[ia0]);
+            }
             acc[k] = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
         }
     }
@@ -193,7 +261,8 @@ int nefct_fdk_preweight(const float* projection_value, float* out, int nv, float
         !std::isfinite(bi) || !std::isfinite(scale))
         return NEFCT_ERR_GEOMETRY;
     if (parker && (!finite_pos(dang_deg) || (direction != 1 && direction != -1))) return NEFCT_ERR_GEOMETRY;
-    hipLaunchKernelGGL(k_fdk_preweight, dim3((na + 255) / 256, nb, nv), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_fdk_preweight, dim3((na + 255) / 256, nv, (nb + PW_ROWS - 1) / PW_ROWS), dim3(256), 0,
+                       (hipStream_t)stream,
                        projection_value, out, nv, na, nb, (double)SAD, (double)da, (double)ai, (double)db,
                        (double)bi, parker, (double)dang_deg, direction, (double)scale);
     NEFCT_CHECK_LAUNCH();
@@ -205,8 +274,20 @@ int nefct_fdk_filter(const float* in, float* out, int rows, int na, const float*
     if (!in || !out || !taps) return NEFCT_ERR_NULL;
     if (rows <= 0 || na <= 0 || na > FMAX) return NEFCT_ERR_SHAPE;
     if (in == out) return NEFCT_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL(k_fdk_filter, dim3((rows + FR - 1) / FR), dim3(256), 0, (hipStream_t)stream, in, out, taps,
-                       rows, na);
+    int logn = 6;
+    while ((1 << logn) < 2 * na - 1) ++logn;
+    const long long pairs = ((long long)rows + 1) / 2;
+    const dim3 g((unsigned)(pairs < 4096 ? pairs : 4096));
+    hipStream_t s = (hipStream_t)stream;
+    switch (logn) {
+    case 6: hipLaunchKernelGGL(k_fdk_filter<6>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    case 7: hipLaunchKernelGGL(k_fdk_filter<7>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    case 8: hipLaunchKernelGGL(k_fdk_filter<8>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    case 9: hipLaunchKernelGGL(k_fdk_filter<9>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    case 10: hipLaunchKernelGGL(k_fdk_filter<10>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    case 11: hipLaunchKernelGGL(k_fdk_filter<11>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    default: hipLaunchKernelGGL(k_fdk_filter<12>, g, dim3(256), 0, s, in, out, taps, (long long)rows, na); break;
+    }
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }
@@ -221,6 +302,7 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
     if (!filtered || !grid || !center || !size || !angles || !image) return NEFCT_ERR_NULL;
     if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
     if ((grid[1] + 3) / 4 > 65535 || (grid[2] + ZC - 1) / ZC > 65535) return NEFCT_ERR_SHAPE;
+    if ((size_t)na * (size_t)nb >= (1ull << 29)) return NEFCT_ERR_SHAPE;   // 32-bit byte offsets within a view
     if (!finite_pos(SID) || !finite_pos(SAD) || !finite_pos(da) || !finite_pos(db) || !finite_pos(size[0]) ||
         !finite_pos(size[1]) || !finite_pos(size[2]) || !std::isfinite(center[0]) || !std::isfinite(center[1]) ||
         !std::isfinite(center[2]) || !std::isfinite(ai) || !std::isfinite(bi))
@@ -233,9 +315,15 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
     const double vx = (double)size[0] / nx, vy = (double)size[1] / ny, vz = (double)size[2] / nz;
     const double x0 = (double)center[0] - 0.5 * size[0], y0 = (double)center[1] - 0.5 * size[1],
                  z0 = (double)center[2] - 0.5 * size[2];
-    hipLaunchKernelGGL(k_fdk_bp, dim3((nx + 63) / 64, (ny + 3) / 4, (nz + ZC - 1) / ZC), dim3(256), 0, s, filtered,
-                       (const double*)trig, offsets, nv, nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID,
-                       (double)SAD, na, (double)da, (double)ai, nb, (double)db, (double)bi, image);
+    const dim3 g((nx + 63) / 64, (ny + 3) / 4, (nz + ZC - 1) / ZC);
+    if (na >= 2)
+        hipLaunchKernelGGL(k_fdk_bp<true>, g, dim3(256), 0, s, filtered, (const double*)trig, offsets, nv, nx, ny, nz,
+                           x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da, (double)ai, nb,
+                           (double)db, (double)bi, image);
+    else
+        hipLaunchKernelGGL(k_fdk_bp<false>, g, dim3(256), 0, s, filtered, (const double*)trig, offsets, nv, nx, ny,
+                           nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da, (double)ai, nb,
+                           (double)db, (double)bi, image);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }

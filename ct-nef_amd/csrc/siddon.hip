@@ -163,17 +163,6 @@ __device__ __forceinline__ void clip_axis(double k, double inv_k, double A, doub
     tout = fmin(tout, t1);
 }
 
-// 1/x in FP64 from the hardware reciprocal and two Newton steps (a few ulp at
-// most, against ~10 instructions for a correctly rounded division).  The
-// forward and the adjoint share this set-up, so they see identical rays.
-__device__ __forceinline__ double rcp64(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
 
 // What the set-up needs of walk axis m and its transverse axes a, b: the
 // volume's extents and voxel sizes and the view's source in voxel units.

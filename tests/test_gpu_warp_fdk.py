@@ -4,6 +4,7 @@ Bars: warp forward/transpose/displacement gradient within 1e-6 relative L2 of
 the FP64 oracle (the sample position is formed in FP32 on both sides);
 FDK stages within 1e-5 relative L2; dot tests of the warp pair to 1e-6.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -120,6 +121,27 @@ def test_fdk_stages_vs_oracle(oracle):
     # backprojection
     bp = F.backproject(s, vol, det, torch.tensor(ang)).cpu().numpy()
     assert _rel(bp, oracle.fdk_backproject(ov, od, ang.astype(np.float64), sino.astype(np.float64))) < 1e-5
+
+
+@pytest.mark.parametrize("na,rows", [(1, 3), (2, 1), (37, 5), (64, 4), (333, 7), (1024, 3), (2048, 2)])
+def test_fdk_filter_arbitrary_taps(na, rows):
+    """nefct_fdk_filter is the linear row convolution with any (2 na - 1) taps
+    (centre na - 1): random non-symmetric taps, odd row counts, every FFT
+    length from 64 to 4096, against the direct sum in FP64."""
+    from nefct_hip import _lib
+    from nefct_hip.ops import _stream_ptr
+    rng = np.random.default_rng(na * 31 + rows)
+    x = rng.standard_normal((rows, na)).astype(np.float32)
+    h = rng.standard_normal(2 * na - 1).astype(np.float32)
+    want = np.stack([np.convolve(r.astype(np.float64), h.astype(np.float64))[na - 1:2 * na - 1] for r in x])
+    xi, hi = torch.tensor(x, device=DEV), torch.tensor(h, device=DEV)
+    out = torch.full_like(xi, float("nan"))
+    L = _lib.load()
+    vp = ctypes.c_void_p
+    st = L.nefct_fdk_filter(vp(xi.data_ptr()), vp(out.data_ptr()), rows, na, vp(hi.data_ptr()),
+                            vp(_stream_ptr(xi.device)))
+    _lib.check(st, "fdk_filter")
+    assert _rel(out.cpu().numpy(), want) < 1e-5
 
 
 def test_fdk_helical_offsets_vs_oracle(oracle):
