@@ -51,6 +51,22 @@ __device__ __forceinline__ bool finite3(float a, float b, float c)
     return fabsf(a) < 1e30f && fabsf(b) < 1e30f && fabsf(c) < 1e30f;
 }
 
+// The two x-corners of a (y, z) row, each weighted, or 0 outside the volume.
+// When both lie in the row (always under the reference rule) they are read as
+// one 8-byte load (4-byte aligned; gfx950 loads need only dword alignment).
+__device__ __forceinline__ float2 row_pair(const float* __restrict__ img, int x0, int y, int z, int nx, int ny, int nz)
+{
+    if ((unsigned)y >= (unsigned)ny || (unsigned)z >= (unsigned)nz) return make_float2(0.0f, 0.0f);
+    const float* row = img + (size_t)nx * ((size_t)y + (size_t)ny * z);
+    if (x0 >= 0 && x0 + 1 < nx) {
+        float2 p;
+        __builtin_memcpy(&p, row + x0, 8);
+        return p;
+    }
+    return make_float2((unsigned)x0 < (unsigned)nx ? row[x0] : 0.0f,
+                       (unsigned)(x0 + 1) < (unsigned)nx ? row[x0 + 1] : 0.0f);
+}
+
 __global__ void k_deform(const float* __restrict__ img, const float* __restrict__ mx, const float* __restrict__ my,
                          const float* __restrict__ mz, int nx, int ny, int nz, int boundary, float* __restrict__ out)
 {
@@ -65,12 +81,11 @@ __global__ void k_deform(const float* __restrict__ img, const float* __restrict_
         const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
         if (boundary != 0 || c.ok) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int x = c.x0 + (q & 1), y = c.y0 + ((q >> 1) & 1), z = c.z0 + (q >> 2);
-                const float w = ((q & 1) ? c.fx : 1.0f - c.fx) * (((q >> 1) & 1) ? c.fy : 1.0f - c.fy) *
-                                ((q >> 2) ? c.fz : 1.0f - c.fz);
-                if (inside(x, y, z, nx, ny, nz))
-                    v = fmaf(img[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)], w, v);
+            for (int q = 0; q < 4; ++q) {
+                const int y = c.y0 + (q & 1), z = c.z0 + (q >> 1);
+                const float2 p = row_pair(img, c.x0, y, z, nx, ny, nz);
+                const float wyz = ((q & 1) ? c.fy : 1.0f - c.fy) * ((q >> 1) ? c.fz : 1.0f - c.fz);
+                v = fmaf(fmaf(c.fx, p.y - p.x, p.x), wyz, v);
             }
         }
     }
@@ -123,16 +138,15 @@ __global__ void k_deform_dvf_grad(const float* __restrict__ img, const float* __
         const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
         if (boundary != 0 || c.ok) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int x = c.x0 + (q & 1), y = c.y0 + ((q >> 1) & 1), z = c.z0 + (q >> 2);
-                if (!inside(x, y, z, nx, ny, nz)) continue;
-                const float v = img[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
-                const float wx = (q & 1) ? c.fx : 1.0f - c.fx, sx = (q & 1) ? 1.0f : -1.0f;
-                const float wy = ((q >> 1) & 1) ? c.fy : 1.0f - c.fy, sy = ((q >> 1) & 1) ? 1.0f : -1.0f;
-                const float wz = (q >> 2) ? c.fz : 1.0f - c.fz, sz = (q >> 2) ? 1.0f : -1.0f;
-                ax = fmaf(v, sx * wy * wz, ax);
-                ay = fmaf(v, wx * sy * wz, ay);
-                az = fmaf(v, wx * wy * sz, az);
+            for (int q = 0; q < 4; ++q) {
+                const int y = c.y0 + (q & 1), z = c.z0 + (q >> 1);
+                const float2 p = row_pair(img, c.x0, y, z, nx, ny, nz);
+                const float wy = (q & 1) ? c.fy : 1.0f - c.fy, sy = (q & 1) ? 1.0f : -1.0f;
+                const float wz = (q >> 1) ? c.fz : 1.0f - c.fz, sz = (q >> 1) ? 1.0f : -1.0f;
+                const float lx = fmaf(c.fx, p.y - p.x, p.x);   // the row interpolated in x
+                ax = fmaf(p.y - p.x, wy * wz, ax);
+                ay = fmaf(lx, sy * wz, ay);
+                az = fmaf(lx, wy * sz, az);
             }
         }
     }
