@@ -74,19 +74,41 @@ __global__ __launch_bounds__(256) void k_fdk_preweight(const float* __restrict__
 // response, then takes two rows at a time as the real and imaginary parts of
 // one complex row (the taps are real, so the two filtered rows come back as
 // the real and imaginary parts): FFT, multiply, inverse FFT through
-// conj(FFT(conj(.))) / N.  FFTs are radix-2 Stockham (self-sorting) in LDS.
+// conj(FFT(conj(.))) / N.  FFTs are radix-4 Stockham (self-sorting) in LDS.
 constexpr int FMAX = 2048;   // longest detector row supported (N <= 4096)
 
+// Forward DFT of buf[a] (length N = 2^k) by Stockham passes, radix 4 and a
+// last radix-2 pass when k is odd; tw[t] = exp(-2 pi i t / N), t < N.
+// Returns the buffer that holds the result.
 template <int N>
 __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
 {
-    constexpr int H2 = N / 2;
-    for (int Ns = 1; Ns < N; Ns <<= 1) {
+    auto cmul = [](float2 u, float2 w) { return make_float2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x); };
+    constexpr int Q = N / 4;
+    int Ns = 1;
+    for (; Ns * 4 <= N; Ns *= 4) {
+        const int step = N / (Ns * 4);
+        for (int j = threadIdx.x; j < Q; j += 256) {
+            const int k = j & (Ns - 1);
+            const float2 v0 = buf[a][j];
+            const float2 v1 = cmul(buf[a][j + Q], tw[k * step]);
+            const float2 v2 = cmul(buf[a][j + 2 * Q], tw[2 * k * step]);
+            const float2 v3 = cmul(buf[a][j + 3 * Q], tw[3 * k * step]);
+            const int d = ((j - k) << 2) + k;
+            buf[a ^ 1][d] = make_float2(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
+            buf[a ^ 1][d + Ns] = make_float2(v0.x + v1.y - v2.x - v3.y, v0.y - v1.x - v2.y + v3.x);
+            buf[a ^ 1][d + 2 * Ns] = make_float2(v0.x - v1.x + v2.x - v3.x, v0.y - v1.y + v2.y - v3.y);
+            buf[a ^ 1][d + 3 * Ns] = make_float2(v0.x - v1.y - v2.x + v3.y, v0.y + v1.x - v2.y - v3.x);
+        }
+        __syncthreads();
+        a ^= 1;
+    }
+    if (Ns < N) {
+        constexpr int H2 = N / 2;
+        const int step = N / (Ns * 2);
         for (int j = threadIdx.x; j < H2; j += 256) {
             const int k = j & (Ns - 1);
-            const float2 v0 = buf[a][j], u1 = buf[a][j + H2];
-            const float2 w = tw[k * (H2 / Ns)];   // exp(-2 pi i k / (2 Ns))
-            const float2 v1 = make_float2(u1.x * w.x - u1.y * w.y, u1.x * w.y + u1.y * w.x);
+            const float2 v0 = buf[a][j], v1 = cmul(buf[a][j + H2], tw[k * step]);
             const int d = ((j - k) << 1) + k;
             buf[a ^ 1][d] = make_float2(v0.x + v1.x, v0.y + v1.y);
             buf[a ^ 1][d + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
@@ -101,12 +123,12 @@ template <int LOGN>
 __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in, float* __restrict__ out,
                                                     const float* __restrict__ h, long long rows, int na)
 {
-    constexpr int N = 1 << LOGN, H2 = N / 2;
+    constexpr int N = 1 << LOGN;
     __shared__ float2 buf[2][N];
     __shared__ float2 resp[N];
-    __shared__ float2 tw[H2];
+    __shared__ float2 tw[N];
     const int tid = threadIdx.x;
-    for (int t = tid; t < H2; t += 256) {
+    for (int t = tid; t < N; t += 256) {
         float sn, cs;
         sincospif(-2.0f * (float)t / (float)N, &sn, &cs);
         tw[t] = make_float2(cs, sn);
@@ -160,7 +182,7 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
 // ZC voxels of the column; the row coordinate is FP32 per voxel.  With
 // PAIR (na >= 2, so ia0 <= na - 2) the two taps of a detector row are read as
 // one 8-byte load (4-byte aligned; gfx950 loads need only dword alignment).
-constexpr int ZC = 16;
+constexpr int ZC = 8;
 
 template <bool PAIR>
 __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, const double* __restrict__ trig,
