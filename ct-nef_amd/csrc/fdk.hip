@@ -208,6 +208,7 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
     const float inv_db = (float)(1.0 / db);
     const float fb0 = (float)(hb / db - bi);
     const float nb1 = (float)(nb - 1);
+    const unsigned rowstep = (nb >= 2) ? 4u * (unsigned)na : 0u;   // byte offset of row ib0 + 1
     for (int iv = 0; iv < nv; ++iv) {
         // view-uniform values first, before the lanes diverge
         const float* qv = q + (size_t)nb * na * iv;
@@ -225,30 +226,41 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
         const float w = wr * wr;
         const float mb = (float)mag * inv_db;
         const int ia1 = min(ia0 + 1, na - 1);
+        // branch-free over the ZC voxels, loads first, so all of them are in
+        // flight together: a row coordinate outside the detector is clamped for
+        // the address and its sample dropped by the select
+        float2 p0[ZC], p1[ZC];
+        float tb[ZC];
+        bool valid[ZC];
+        const float fbv = fmaf(-off, mb, fb0);   // row coordinate of z = 0
 #pragma unroll
         for (int k = 0; k < ZC; ++k) {
-            const float fb = fmaf(zf[k] - off, mb, fb0);
-            if (!(k < kn && fb >= 0.0f && fb <= nb1)) continue;
-            const int ib0 = min((int)floorf(fb), max(nb - 2, 0));
+            const float fb = fmaf(zf[k], mb, fbv);
+            valid[k] = k < kn && fb >= 0.0f && fb <= nb1;
+            const float fbc = fminf(fmaxf(fb, 0.0f), nb1);
+            const int ib0 = min((int)floorf(fbc), max(nb - 2, 0));
             const int ib1 = min(ib0 + 1, nb - 1);
-            const float tb = fb - (float)ib0;
-            const float* r0p = qv + (size_t)ib0 * na;
-            const float* r1p = qv + (size_t)ib1 * na;
-            float top, bot;
+            tb[k] = fbc - (float)ib0;
             if (PAIR) {
-                // 32-bit byte offsets from the view's base (a view has < 2^29 cells)
-                const unsigned o0 = 4u * (unsigned)(ib0 * na + ia0), o1 = o0 + 4u * (unsigned)((ib1 - ib0) * na);
+                // 32-bit byte offsets from the view's base (a view has < 2^29
+                // cells; 24-bit multiplies, na and nb < 2^24)
+                const unsigned o0 = 4u * (unsigned)(__mul24(ib0, na) + ia0), o1 = o0 + rowstep;
                 const char* qb = reinterpret_cast<const char*>(qv);
-                float2 p0, p1;
-                __builtin_memcpy(&p0, qb + o0, 8);
-                __builtin_memcpy(&p1, qb + o1, 8);
-                top = fmaf(ta, p0.y - p0.x, p0.x);
-                bot = fmaf(ta, p1.y - p1.x, p1.x);
+                __builtin_memcpy(&p0[k], qb + o0, 8);
+                __builtin_memcpy(&p1[k], qb + o1, 8);
             } else {
-                top = fmaf(ta, r0p[ia1] - r0p[ia0], r0p[ia0]);
-                bot = fmaf(ta, r1p[ia1] - r1p[ia0], r1p[ia0]);
+                const float* r0p = qv + (size_t)ib0 * na;
+                const float* r1p = qv + (size_t)ib1 * na;
+                p0[k] = make_float2(r0p[ia0], r0p[ia1]);
+                p1[k] = make_float2(r1p[ia0], r1p[ia1]);
             }
-            acc[k] = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < ZC; ++k) {
+            const float top = fmaf(ta, p0[k].y - p0[k].x, p0[k].x);
+            const float bot = fmaf(ta, p1[k].y - p1[k].x, p1[k].x);
+            const float sum = fmaf(w, fmaf(tb[k], bot - top, top), acc[k]);
+            acc[k] = valid[k] ? sum : acc[k];
         }
     }
 #pragma unroll
@@ -324,7 +336,8 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
     if (!filtered || !grid || !center || !size || !angles || !image) return NEFCT_ERR_NULL;
     if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
     if ((grid[1] + 3) / 4 > 65535 || (grid[2] + ZC - 1) / ZC > 65535) return NEFCT_ERR_SHAPE;
-    if ((size_t)na * (size_t)nb >= (1ull << 29)) return NEFCT_ERR_SHAPE;   // 32-bit byte offsets within a view
+    // 32-bit byte offsets within a view, 24-bit row multiplies
+    if ((size_t)na * (size_t)nb >= (1ull << 29) || na >= (1 << 24) || nb >= (1 << 24)) return NEFCT_ERR_SHAPE;
     if (!finite_pos(SID) || !finite_pos(SAD) || !finite_pos(da) || !finite_pos(db) || !finite_pos(size[0]) ||
         !finite_pos(size[1]) || !finite_pos(size[2]) || !std::isfinite(center[0]) || !std::isfinite(center[1]) ||
         !std::isfinite(center[2]) || !std::isfinite(ai) || !std::isfinite(bi))
