@@ -1124,16 +1124,17 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             ib = ib0 + (int)rr;
             return e < ncell;
         };
-        const size_t view_base = (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
+        // the view's sinogram; a cell's offset in it fits 32 bits (na nb < 2^31)
+        const float* __restrict__ pvv = pv + (size_t)P.G.na * (size_t)P.G.nb * (size_t)iv;
         int ia_n = 0, ib_n = 0;
         bool ok_n = nk > 0 && cell_of(0, ia_n, ib_n);
-        float pv_n = ok_n ? pv[view_base + (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n] : 0.0f;
+        float pv_n = ok_n ? pvv[(unsigned)(ia_n + P.G.na * ib_n)] : 0.0f;
         for (int k = 0; k < nk; ++k) {
             const int ia = ia_n, ib = ib_n;
             const bool ok = ok_n;
             const float pval = pv_n;
             ok_n = k + 1 < nk && cell_of(k + 1, ia_n, ib_n);
-            if (ok_n) pv_n = pv[view_base + (size_t)ia_n + (size_t)P.G.na * (size_t)ib_n];
+            if (ok_n) pv_n = pvv[(unsigned)(ia_n + P.G.na * ib_n)];
             if (!ok) continue;
             Ray r;
             // the set-up's per-axis values from the LDS tables (the same
@@ -1526,6 +1527,7 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
     if (!grid || !center || !size || !angles) return NEFCT_ERR_NULL;
     if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
     if (nb > 65535 * BY || nv > 65535) return NEFCT_ERR_SHAPE;
+    if ((size_t)na * (size_t)nb >= 0x7fffffffull) return NEFCT_ERR_SHAPE;   // 32-bit offsets within a view
     if (grid[0] > 16384 || grid[1] > 16384 || grid[2] > 16384) return NEFCT_ERR_SHAPE;
     const int grid_arr[3] = {grid[0], grid[1], grid[2]};
     const WsPlan p = plan_ws(grid_arr, nv);
