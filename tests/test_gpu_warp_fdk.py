@@ -123,6 +123,27 @@ def test_fdk_stages_vs_oracle(oracle):
     assert _rel(bp, oracle.fdk_backproject(ov, od, ang.astype(np.float64), sino.astype(np.float64))) < 1e-5
 
 
+@pytest.mark.parametrize("na", [2, 7])
+def test_fdk_backproject_narrow_detector_partial_z_chunk(oracle, na):
+    """Backprojection edge cases: a detector two or seven columns wide (the
+    two taps of a row read as one 8-byte load at the row's end) and nz = 13,
+    which leaves a partial chunk of z voxels for the last threads."""
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    from nefct_hip import fdk as F
+    vol = VolumeGeometry.cube((9, 11, 13), 1.0)
+    det = ConeGeometry.flat(60.0, 90.0, na, 3.0, 6, 4.0)
+    ov = oracle.vol_struct(vol.grid, vol.center, vol.size)
+    od = oracle.det_struct(0, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi)
+    rng = np.random.default_rng(na)
+    nv = 12
+    q = rng.standard_normal((nv, det.nb, det.na)).astype(np.float32)
+    ang = (2 * np.pi * np.arange(nv) / nv + 0.1).astype(np.float32)
+    bp = F.backproject(torch.tensor(q, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+    want = oracle.fdk_backproject(ov, od, ang.astype(np.float64), q.astype(np.float64))
+    assert np.count_nonzero(want) > 0
+    assert _rel(bp, want) < 1e-5
+
+
 @pytest.mark.parametrize("na,rows", [(1, 3), (2, 1), (37, 5), (64, 4), (333, 7), (1024, 3), (2048, 2)])
 def test_fdk_filter_arbitrary_taps(na, rows):
     """nefct_fdk_filter is the linear row convolution with any (2 na - 1) taps
