@@ -237,8 +237,8 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
         for (int k = 0; k < ZC; ++k) {
             const float fb = fmaf(zf[k], mb, fbv);
             valid[k] = k < kn && fb >= 0.0f && fb <= nb1;
-            const float fbc = fminf(fmaxf(fb, 0.0f), nb1);
-            const int ib0 = min((int)floorf(fbc), max(nb - 2, 0));
+            const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
+            const int ib0 = min((int)fbc, max(nb - 2, 0));   // fbc >= 0: truncation is floor
             const int ib1 = min(ib0 + 1, nb - 1);
             tb[k] = fbc - (float)ib0;
             if (PAIR) {
