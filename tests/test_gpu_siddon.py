@@ -257,6 +257,39 @@ def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
         assert _relerr(got_r, ref_r) < 1e-5, (algo, name, "reference mode", _relerr(got_r, ref_r))
 
 
+def test_adjoint_many_views(oracle):
+    """1100 views: the tile adjoint fills its per-view LDS tables (detector
+    shadows, sources) 512 views at a time, so this crosses two chunk
+    boundaries.  Against the FP64 transpose (non-negative and mixed-sign
+    data, which take different deposit paths), the dot test, and the batched
+    adjoint of the same problem."""
+    nh = _nh()
+    from nefct_hip import ConeGeometry, VolumeGeometry
+    from nefct_hip.batch import SiddonBatch
+    vol = VolumeGeometry.cube((20, 22, 18), 1.5)
+    det = ConeGeometry.flat(80.0, 120.0, 40, 1.3, 30, 1.2)
+    nv = 1100
+    ang = (2 * math.pi * np.arange(nv) / nv + 0.02).astype(np.float32)
+    ov = oracle.vol_struct(vol.grid, vol.center, vol.size)
+    od = oracle.det_struct(det.cyl, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi)
+    rng = np.random.default_rng(41)
+    for y in (rng.uniform(0, 1, size=(nv, det.nb, det.na)), rng.standard_normal((nv, det.nb, det.na))):
+        y = y.astype(np.float32)
+        got = nh.backproject(torch.tensor(y, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
+        ref = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
+        assert _relerr(got, ref) < 1e-5, _relerr(got, ref)
+    x = torch.randn(vol.shape_zyx, device=DEV, dtype=torch.float32)
+    yt = torch.randn((nv, det.nb, det.na), device=DEV, dtype=torch.float32)
+    lhs = torch.sum(nh.project(x, vol, det, torch.tensor(ang)).double() * yt.double()).item()
+    rhs = torch.sum(x.double() * nh.backproject(yt, vol, det, torch.tensor(ang)).double()).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs), (lhs, rhs)
+    b = SiddonBatch([(vol, det, ang)])
+    b.sinograms[0].copy_(yt)
+    b.backproject()
+    ref = nh.backproject(yt, vol, det, torch.tensor(ang))
+    assert torch.allclose(b.volumes[0], ref, rtol=1e-5, atol=1e-5 * ref.abs().max().item())
+
+
 @pytest.mark.parametrize("shift", [0.0, 1.0 / 256])
 def test_boundary_aligned_rays(oracle, shift):
     """Rays on (or within 1/256 voxel of) voxel faces: views at multiples of
