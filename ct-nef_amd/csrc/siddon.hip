@@ -934,7 +934,8 @@ __device__ __forceinline__ void lds_deposit(char* base, int byte_off, float v)
     }
 }
 
-// Fixed-point deposit of a nonzero quantum q whose sign word is hi (0 or -1).
+// Fixed-point deposit of quantum q with sign word hi (0 or -1); with hi = -1
+// q must be nonzero (a zero quantum would carry the sign word alone).
 __device__ __forceinline__ void lds_deposit_q(char* base, int byte_off, int q, int hi)
 {
     const unsigned long long v = ((unsigned long long)(unsigned)hi << 32) | (unsigned)q;
