@@ -396,7 +396,7 @@ __device__ __forceinline__ bool cell_in(const Walk& w, int ca, int cb)
 //
 // The three gathers of a slab are issued as three predicated loads whose
 // results are consumed only after all of them, so a wavefront has all of a
-// slab's loads (two slabs' with the unroll) in flight at once.
+// slab's loads (up to four slabs' with the unroll) in flight at once.
 template <int MODE, bool REF, bool CHECKED, bool STEP = false>
 __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Walk& w,
                           __amdgpu_buffer_rsrc_t rs, float val)
@@ -470,9 +470,11 @@ __device__ float walk_ray(const Ray& r, bool mine, int tmin, int tmax, const Wal
         const int m0 = __builtin_amdgcn_readfirstlane(wave_max_i32(mine ? r.s0 : tmin));
         const int m1 = __builtin_amdgcn_readfirstlane(wave_min_i32(mine ? r.s1 : tmax));
         const int ta = min(max(m0, tmin), tmax), tb2 = max(min(m1, tmax), ta);
+        // the middle part is unrolled by 4: four slabs' twelve gathers in
+        // flight per trip (2: 5 % slower at 720 views, 8: 25 % slower)
 #pragma unroll 2
         for (int t = tmin; t < ta; ++t) slab(t, std::true_type{});
-#pragma unroll 2
+#pragma unroll 4
         for (int t = ta; t < tb2; ++t) slab(t, std::false_type{});
 #pragma unroll 2
         for (int t = tb2; t < tmax; ++t) slab(t, std::true_type{});
