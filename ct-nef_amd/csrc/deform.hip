@@ -12,7 +12,7 @@
 // m is in voxel units.  Boundary rule 0 (reference): the voxel is written only
 // when every base index floor(p) lies in [1, n-2] (deform.cu.cc:31-33), else 0.
 // Rule 1 (zero padding): every corner inside the volume contributes.
-// One lane per output voxel (the forward: two, in neighbouring y rows), x
+// One lane per output voxel (forward and field gradient: two, in neighbouring y rows), x
 // fastest: the three displacement reads and the write are coalesced, the 8
 // gathers hit L2 (neighbouring lanes share corners).
 #include "common.h"
@@ -139,33 +139,46 @@ __global__ void k_deform_dvf_grad(const float* __restrict__ img, const float* __
                                   const float* __restrict__ mz, int nx, int ny, int nz, int boundary,
                                   float* __restrict__ gx, float* __restrict__ gy, float* __restrict__ gz)
 {
+    // two voxels per lane in neighbouring y rows, as in k_deform
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y;
     const int iz = blockIdx.z;
     if (ix >= nx) return;
-    const size_t id = (size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz);
-    const float gv = g[id];
-    const float dx = mx[id], dy = my[id], dz = mz[id];
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    if (gv != 0.0f && finite3(dx, dy, dz)) {
-        const Corners c = corners_of(ix, iy, iz, dx, dy, dz, nx, ny, nz);
-        if (boundary != 0 || c.ok) {
+    float ax[2], ay[2], az[2], gv[2];
+    size_t ids[2];
+    bool in[2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int y = c.y0 + (q & 1), z = c.z0 + (q >> 1);
-                const float2 p = row_pair(img, c.x0, y, z, nx, ny, nz);
-                const float wy = (q & 1) ? c.fy : 1.0f - c.fy, sy = (q & 1) ? 1.0f : -1.0f;
-                const float wz = (q >> 1) ? c.fz : 1.0f - c.fz, sz = (q >> 1) ? 1.0f : -1.0f;
-                const float lx = fmaf(c.fx, p.y - p.x, p.x);   // the row interpolated in x
-                ax = fmaf(p.y - p.x, wy * wz, ax);
-                ay = fmaf(lx, sy * wz, ay);
-                az = fmaf(lx, wy * sz, az);
+    for (int k = 0; k < 2; ++k) {
+        const int iy = blockIdx.y * 2 + k;
+        in[k] = iy < ny;
+        const int iyc = in[k] ? iy : ny - 1;   // the row past an odd ny reads the last row, writes nothing
+        ids[k] = (size_t)ix + (size_t)nx * ((size_t)iyc + (size_t)ny * iz);
+        gv[k] = g[ids[k]];
+        const float dx = mx[ids[k]], dy = my[ids[k]], dz = mz[ids[k]];
+        ax[k] = ay[k] = az[k] = 0.0f;
+        if (gv[k] != 0.0f && finite3(dx, dy, dz)) {
+            const Corners c = corners_of(ix, iyc, iz, dx, dy, dz, nx, ny, nz);
+            if (boundary != 0 || c.ok) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int y = c.y0 + (q & 1), z = c.z0 + (q >> 1);
+                    const float2 p = row_pair(img, c.x0, y, z, nx, ny, nz);
+                    const float wy = (q & 1) ? c.fy : 1.0f - c.fy, sy = (q & 1) ? 1.0f : -1.0f;
+                    const float wz = (q >> 1) ? c.fz : 1.0f - c.fz, sz = (q >> 1) ? 1.0f : -1.0f;
+                    const float lx = fmaf(c.fx, p.y - p.x, p.x);   // the row interpolated in x
+                    ax[k] = fmaf(p.y - p.x, wy * wz, ax[k]);
+                    ay[k] = fmaf(lx, sy * wz, ay[k]);
+                    az[k] = fmaf(lx, wy * sz, az[k]);
+                }
             }
         }
     }
-    gx[id] = ax * gv;
-    gy[id] = ay * gv;
-    gz[id] = az * gv;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (!in[k]) continue;
+        gx[ids[k]] = ax[k] * gv[k];
+        gy[ids[k]] = ay[k] * gv[k];
+        gz[ids[k]] = az[k] * gv[k];
+    }
 }
 
 __global__ void k_fill0(float* __restrict__ p, size_t n)
@@ -231,7 +244,7 @@ int nefct_deform_dvf_grad(const float* image, const float* grad_out, const float
     if (!image || !grad_out || !mx || !my || !mz || !grad_mx || !grad_my || !grad_mz) return NEFCT_ERR_NULL;
     if (boundary != 0 && boundary != 1) return NEFCT_ERR_UNSUPPORTED;
     const int nx = grid[0], ny = grid[1], nz = grid[2];
-    hipLaunchKernelGGL(k_deform_dvf_grad, dim3((nx + 255) / 256, ny, nz), dim3(256), 0, (hipStream_t)stream, image,
+    hipLaunchKernelGGL(k_deform_dvf_grad, dim3((nx + 255) / 256, (ny + 1) / 2, nz), dim3(256), 0, (hipStream_t)stream, image,
                        grad_out, mx, my, mz, nx, ny, nz, boundary, grad_mx, grad_my, grad_mz);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
