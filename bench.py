@@ -77,7 +77,7 @@ def batched_leg(dev, iters, log_every=True):
     probs = c5()
     log(f"[bench] batched leg: {len(probs)} problems, building phantoms and warp fields")
     bases = [ellipsoid_phantom(p.vol, p.phantom_seed, 10, device=dev) for p in probs]
-    dvfs = [gaussian_dvf(p.vol.shape_zyx, p.dvf_seed, device=dev) for p in probs]
+    dvfs = [gaussian_dvf(p.vol.shape_zyx, p.dvf_seed, voxel_mm=p.vol.voxel, device=dev) for p in probs]
     angs = [torch.tensor(p.angles, device=dev) for p in probs]
     batch = SiddonBatch([(p.vol, p.det, p.angles) for p in probs], device=dev)
     g = torch.Generator(device=dev).manual_seed(55)
@@ -197,7 +197,7 @@ def warp_leg(wl, img, reps=3):
     from nefct_hip.deform import deform_adjoint_op, deform_dvf_grad_op, deform_op
     from nefct_hip.phantom import gaussian_dvf
     nvox = img.numel()
-    dvf = gaussian_dvf(wl.vol.shape_zyx, 7, voxel_mm=wl.vol.voxel[0], device=img.device).float().contiguous()
+    dvf = gaussian_dvf(wl.vol.shape_zyx, 7, voxel_mm=wl.vol.voxel, device=img.device).float().contiguous()
     g = torch.ones_like(img)
     t_f, _ = _timed(lambda: deform_op(img, dvf[0], dvf[1], dvf[2], 1), reps)
     t_a, _ = _timed(lambda: deform_adjoint_op(g, dvf[0], dvf[1], dvf[2], 1), reps)

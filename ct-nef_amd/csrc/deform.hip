@@ -31,9 +31,14 @@ __device__ __forceinline__ Corners corners_of(int ix, int iy, int iz, float mx, 
     Corners c;
     const float px = (float)ix + mx, py = (float)iy + my, pz = (float)iz + mz;
     const float flx = floorf(px), fly = floorf(py), flz = floorf(pz);
-    c.x0 = (int)flx;
-    c.y0 = (int)fly;
-    c.z0 = (int)flz;
+    // The base index is clamped to [-2, n] before the conversion: a position
+    // that far outside has both corners outside the volume, as it has before
+    // the clamp, and a displacement of 2^31 voxels or more would otherwise
+    // overflow the conversion (finite3 lets |m| < 1e30 through).  The weights
+    // keep the unclamped floor, so they stay in [0, 1].
+    c.x0 = (int)fminf(fmaxf(flx, -2.0f), (float)nx);
+    c.y0 = (int)fminf(fmaxf(fly, -2.0f), (float)ny);
+    c.z0 = (int)fminf(fmaxf(flz, -2.0f), (float)nz);
     c.fx = px - flx;
     c.fy = py - fly;
     c.fz = pz - flz;

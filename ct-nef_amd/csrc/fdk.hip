@@ -337,7 +337,8 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
     if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
     if ((grid[1] + 3) / 4 > 65535 || (grid[2] + ZC - 1) / ZC > 65535) return NEFCT_ERR_SHAPE;
     // 32-bit byte offsets within a view, 24-bit row multiplies
-    if ((size_t)na * (size_t)nb >= (1ull << 29) || na >= (1 << 24) || nb >= (1 << 24)) return NEFCT_ERR_SHAPE;
+    // na, nb < 2^23: __mul24 takes signed 24-bit operands (row offsets ib na < 2^29)
+    if ((size_t)na * (size_t)nb >= (1ull << 29) || na >= (1 << 23) || nb >= (1 << 23)) return NEFCT_ERR_SHAPE;
     if (!finite_pos(SID) || !finite_pos(SAD) || !finite_pos(da) || !finite_pos(db) || !finite_pos(size[0]) ||
         !finite_pos(size[1]) || !finite_pos(size[2]) || !std::isfinite(center[0]) || !std::isfinite(center[1]) ||
         !std::isfinite(center[2]) || !std::isfinite(ai) || !std::isfinite(bi))

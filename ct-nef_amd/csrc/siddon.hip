@@ -84,7 +84,7 @@ __device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed
 // This file's device code is compiled without NaN semantics
 // (-Xarch_device -fno-honor-nans), which lets (int)floorf(x) become one
 // v_cvt_flr_i32_f32 (~5 % of the walk loops).  Nothing here relies on NaN
-// arithmetic: tests for non-finite data work on the bits (fixed_scale,
+// arithmetic: tests for non-finite data work on the bits (fixed_exp,
 // the fixed-point write-out, the window adjoint's flush), and walk and
 // set-up values are finite by construction.
 __device__ __forceinline__ int cell_floor(float x) { return (int)floorf(x); }
@@ -908,17 +908,22 @@ struct TileAcc {
 template <bool FIX>
 using acc_t = typename std::conditional<FIX, unsigned long long, float>::type;
 
-// Power-of-two fixed-point scale from the sinogram's max |value| (as float
-// bits) and the bound on the per-slab ray length; 0 marks non-finite input.
-__device__ __forceinline__ float fixed_scale(unsigned maxbits, float sbound)
+// Exponent e of the power-of-two fixed-point scale 2^e: the largest e with
+// max |projection| * sbound * 2^e <= 2^30, from the sinogram's max |value|
+// (as float bits) and the bound on the per-slab ray length.  Computed in FP64:
+// for tiny sinograms 2^e lies beyond the FP32 range (2^30 / 1e-32 ~ 2^136),
+// so contributions are scaled by ldexpf(x, e), never by a float 2^e.
+// FIX_POISON marks non-finite input; an all-zero sinogram gives 0.
+constexpr int FIX_POISON = -0x40000000;
+__device__ __forceinline__ int fixed_exp(unsigned maxbits, float sbound)
 {
     // Inf or NaN somewhere (tested on the bits: NaN and Inf sort above 3e38)
-    if (maxbits > __builtin_bit_cast(unsigned, 3.0e38f)) return 0.0f;
-    const float big = __uint_as_float(maxbits) * sbound;
-    if (!(big > 0.0f)) return 1.0f;             // all zero
+    if (maxbits > __builtin_bit_cast(unsigned, 3.0e38f)) return FIX_POISON;
+    const double big = (double)__uint_as_float(maxbits) * (double)sbound;
+    if (!(big > 0.0)) return 0;                 // all zero
     int e;
-    frexpf(1073741824.0f / big, &e);            // 2^30 / big = m 2^e, m in [0.5, 1)
-    return ldexpf(1.0f, e - 1);                 // largest power of two <= 2^30 / big
+    frexp(1073741824.0 / big, &e);              // 2^30 / big = m 2^e, m in [0.5, 1)
+    return e - 1;                               // 2^(e-1) <= 2^30 / big
 }
 
 // Adds contribution v to the LDS word at byte_off.  Fixed point: v already
@@ -1030,7 +1035,7 @@ struct BlockAxis {
 template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
                                                int bx, int by, int bz, TileAcc<TX, TY, TZ, acc_t<FIX>>& acc,
-                                               int4* __restrict__ shadows, float S, bool accumulate)
+                                               int4* __restrict__ shadows, int es, bool accumulate)
 {
     using T = acc_t<FIX>;
     constexpr int SX = TileAcc<TX, TY, TZ, T>::SX;
@@ -1195,7 +1200,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 t1 = min(t1, ceil_clamp(tb, bm0 - 2, bm1) + SLACK);
             }
             if (t0 >= t1) continue;
-            const float sv = FIX ? (pval * r.scale) * S : pval * r.scale;   // fixed point: scaled once per ray
+            const float sv = FIX ? ldexpf(pval * r.scale, es) : pval * r.scale;   // fixed point: scaled once per ray
             const Walk w = walk_of(m, P.V, P.Ly);
             // LDS byte offset of centred transverse cell (ca, cb) at slab t:
             // (ca Sa + cb Sb + t St + base) * ES, with the strides of the walk
@@ -1343,10 +1348,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         }
     }
     __syncthreads();
-    // S is a power of two, or 0 for a non-finite sinogram: then every voxel is
-    // written as a NaN bit pattern (an integer store)
-    const bool poisoned = FIX && S == 0.0f;
-    const float inv_s = FIX ? 1.0f / (poisoned ? 1.0f : S) : 1.0f;
+    // the scale is 2^es, or FIX_POISON for a non-finite sinogram: then every
+    // voxel is written as a NaN bit pattern (an integer store)
+    const bool poisoned = FIX && es == FIX_POISON;
+    const double inv_s = (FIX && !poisoned) ? ldexp(1.0, -es) : 1.0;
     for (int i = tid; i < TX * TY * TZ; i += 64 * NW) {
         const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
         if (lx < ext[0] && ly < ext[1] && lz < ext[2]) {
@@ -1357,7 +1362,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             }
             float v;
             if (FIX)
-                v = (float)((double)(long long)acc.at(lx, ly, lz) * (double)inv_s);
+                v = (float)((double)(long long)acc.at(lx, ly, lz) * inv_s);
             else
                 v = (float)acc.at(lx, ly, lz);
             image[o] = accumulate ? image[o] + v : v;
@@ -1372,8 +1377,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_siddon_adj_tile(Launch P, const 
 {
     __shared__ TileAcc<TX, TY, TZ, acc_t<FIX>> acc;
     __shared__ int4 shadows[SHV];
-    const float S = FIX ? fixed_scale(*maxbits, sbound) : 1.0f;
-    adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, shadows, S,
+    const int es = FIX ? fixed_exp(*maxbits, sbound) : 0;
+    adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, shadows, es,
                                                  false);
 }
 
@@ -1631,6 +1636,8 @@ int run_forward(const float* image, const int grid[3], const float center[3], co
     return NEFCT_OK;
 }
 
+void launch_adjoint_tiles(const Launch& L, const float* pv, float* image, unsigned* maxbits, hipStream_t s);
+
 int run_adjoint(const float* pv, const int grid[3], const float center[3], const float size[3], const float* angles,
                 int nv, float SID, float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
                 float* image, unsigned flags, void* workspace, size_t wsbytes, void* stream)
@@ -1659,13 +1666,25 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
                            (const float*)(ws + p.xf_off), (const float*)(ws + p.yf_off), nx, ny, nz, p.NXP, p.NYP,
                            image);
     } else {
-        // Volume blocks own their voxels.  Fixed-point accumulation first
-        // needs max |projection| (into a word of the otherwise unused
-        // padded-copy region of the workspace).
+        // the max |projection| word goes to the otherwise unused padded-copy
+        // region of the workspace
+        launch_adjoint_tiles(L, pv, image, (unsigned*)(ws + p.xf_off), s);
+    }
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+// The volume-tile adjoint (every variant but the slab windows): volume blocks
+// own their voxels.  L.trig must hold the view table.  Fixed-point
+// accumulation first needs max |projection|, into *maxbits (cleared here).
+void launch_adjoint_tiles(const Launch& L, const float* pv, float* image, unsigned* maxbits, hipStream_t s)
+{
+    const unsigned flags = L.flags;
+    const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
+    {
         const bool fix = !(flags & NEFCT_F_ADJ_FLOAT_ATOMIC);
-        unsigned* maxbits = (unsigned*)(ws + p.xf_off);
         if (fix) {
-            const size_t n = (size_t)nv * (size_t)na * (size_t)nb;
+            const size_t n = (size_t)L.nv * (size_t)L.G.na * (size_t)L.G.nb;
             launch_zero((float*)maxbits, 1, s);
             const size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
             hipLaunchKernelGGL(k_absmax, dim3((unsigned)blocks), dim3(256), 0, s, pv, n, maxbits);
@@ -1696,8 +1715,6 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
             else launch_tile<32, 32, 16, 16, 1>(ref, fix, L, pv, image, maxbits, sb, s);
         }
     }
-    NEFCT_CHECK_LAUNCH();
-    return NEFCT_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1853,9 +1870,9 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
     const int by = __builtin_amdgcn_readfirstlane((local / pr.blocks[0]) % pr.blocks[1]);
     const int bz = __builtin_amdgcn_readfirstlane(local / (pr.blocks[0] * pr.blocks[1]));
     const Launch& L = batch_launches(ws_base)[p];
-    const float S = fixed_scale(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
+    const int es = fixed_exp(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
     adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
-                                                     shadows, S, false);
+                                                     shadows, es, false);
 }
 
 }  // namespace

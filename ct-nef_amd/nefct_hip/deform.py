@@ -22,17 +22,23 @@ _vp = ctypes.c_void_p
 
 
 def _grid(t):
-    nz, ny, nx = t.shape[-3:]
+    if t.dim() != 3:
+        raise _lib.NefctError(f"volumes are [nz, ny, nx]; got shape {tuple(t.shape)}")
+    nz, ny, nx = t.shape
     return [int(nx), int(ny), int(nz)]
+
+
+def _same_shape(ref, *others):
+    for m in others:
+        if m.shape != ref.shape:
+            raise _lib.NefctError(f"shape {tuple(m.shape)} != {tuple(ref.shape)}")
 
 
 @torch.library.custom_op("nefct::deform", mutates_args=())
 def deform_op(image: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor,
               boundary: int) -> torch.Tensor:
     _need_device(image, mx, my, mz)
-    for m in (mx, my, mz):
-        if m.shape != image.shape:
-            raise _lib.NefctError(f"displacement shape {tuple(m.shape)} != image shape {tuple(image.shape)}")
+    _same_shape(image, mx, my, mz)
     image, mx, my, mz = (t.contiguous() for t in (image, mx, my, mz))
     out = torch.empty_like(image)
     st = _lib.load().nefct_deform(_ptr(image), _ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(image)), _ptr(out),
@@ -50,6 +56,7 @@ def _(image, mx, my, mz, boundary):
 def deform_adjoint_op(grad: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor,
                       boundary: int) -> torch.Tensor:
     _need_device(grad, mx, my, mz)
+    _same_shape(grad, mx, my, mz)
     grad, mx, my, mz = (t.contiguous() for t in (grad, mx, my, mz))
     out = torch.empty_like(grad)
     st = _lib.load().nefct_deform_adjoint(_ptr(grad), _ptr(mx), _ptr(my), _ptr(mz), _lib.int3(_grid(grad)),
@@ -68,6 +75,7 @@ def deform_dvf_grad_op(image: torch.Tensor, grad: torch.Tensor, mx: torch.Tensor
                        mz: torch.Tensor, boundary: int) -> torch.Tensor:
     """Returns [3, *shape]: d<grad, deform(image, m)>/d(mx, my, mz)."""
     _need_device(image, grad, mx, my, mz)
+    _same_shape(image, grad, mx, my, mz)
     image, grad, mx, my, mz = (t.contiguous() for t in (image, grad, mx, my, mz))
     out = torch.empty((3,) + tuple(image.shape), dtype=torch.float32, device=image.device)
     st = _lib.load().nefct_deform_dvf_grad(_ptr(image), _ptr(grad), _ptr(mx), _ptr(my), _ptr(mz),
@@ -119,12 +127,6 @@ def warp(image, dvf, boundary: int = 1):
 # Texture-semantics warps (nefct/cc/deform/deform_tex.cc:12-66): clamp-to-edge
 # trilinear sampling with the sample coordinate i + 1/2 at voxel i.
 # --------------------------------------------------------------------------
-def _same_shape(image, *fields):
-    for m in fields:
-        if m.shape != image.shape:
-            raise _lib.NefctError(f"displacement shape {tuple(m.shape)} != image shape {tuple(image.shape)}")
-
-
 @torch.library.custom_op("nefct::deform_tex", mutates_args=())
 def deform_tex_op(image: torch.Tensor, mx: torch.Tensor, my: torch.Tensor, mz: torch.Tensor) -> torch.Tensor:
     _need_device(image, mx, my, mz)

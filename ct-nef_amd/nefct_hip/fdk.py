@@ -83,6 +83,8 @@ def preweight(sino: torch.Tensor, det: ConeGeometry, parker: bool = False, dang_
     if det.cyl:
         raise _lib.NefctError("FDK weighting is implemented for flat detectors only")
     sino = sino.contiguous()
+    if sino.dim() != 3 or tuple(sino.shape[1:]) != (det.nb, det.na):
+        raise _lib.NefctError(f"projections have shape {tuple(sino.shape)}, expected (nv, {det.nb}, {det.na})")
     nv = sino.shape[0]
     out = torch.empty_like(sino)
     st = _lib.load().nefct_fdk_preweight(_ptr(sino), _ptr(out), nv, det.SID, det.SAD, det.na, det.da, det.ai,
@@ -113,10 +115,14 @@ def backproject(filtered: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, 
     ang = angles if torch.is_tensor(angles) else torch.tensor(np.asarray(angles, np.float32))
     ang = ang.to(device=dev, dtype=torch.float32).contiguous()
     nv = ang.numel()
+    if filtered.numel() != nv * det.nb * det.na:
+        raise _lib.NefctError(f"projections have {filtered.numel()} elements, expected {nv}x{det.nb}x{det.na}")
     off = None
     if offsets is not None:
         off = (offsets if torch.is_tensor(offsets) else torch.tensor(np.asarray(offsets, np.float32)))
         off = off.to(device=dev, dtype=torch.float32).contiguous()
+        if off.numel() != nv:
+            raise _lib.NefctError(f"{off.numel()} offsets for {nv} views")
     L = _lib.load()
     wsb = int(L.nefct_fdk_workspace_size(nv))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
@@ -137,6 +143,8 @@ def fdk(sino: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, park
     i at ((180/dang - nv)/2 + i + 1) dang degrees, ParkerWeight.m:9)."""
     ang = np.asarray(angles.cpu() if torch.is_tensor(angles) else angles, np.float64)
     nv = len(ang)
+    if sino.dim() != 3 or sino.shape[0] != nv:
+        raise _lib.NefctError(f"{nv} angles for projections of shape {tuple(sino.shape)}")
     dang = abs(float(ang[1] - ang[0])) if nv > 1 else 2 * math.pi
     if parker and nv > 1:
         direction = 1 if ang[1] > ang[0] else -1

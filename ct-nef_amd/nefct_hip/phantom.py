@@ -70,8 +70,13 @@ def ellipsoid_phantom(vol: VolumeGeometry, seed: int, count: int, device="cpu", 
 
 
 def gaussian_dvf(shape_zyx, seed: int, voxel_mm=1.0, bumps=8, device="cpu"):
-    """Smooth displacement field [3, nz, ny, nx] (mm, z component only): sum of
-    Gaussian bumps, sigma U[20, 40] voxels, peak U[0, 10] mm, centres uniform."""
+    """Smooth displacement field [3, nz, ny, nx] in voxel units (z component
+    only; the warp takes voxel units): sum of Gaussian bumps, sigma U[20, 40]
+    voxels, peak U[0, 10] mm, centres uniform.  ``voxel_mm`` is the z voxel
+    size (a number, or the (x, y, z) voxel sizes) that turns mm into voxels."""
+    vz = float(voxel_mm[2] if isinstance(voxel_mm, (tuple, list)) else voxel_mm)
+    if not vz > 0:
+        raise ValueError("voxel_mm must be positive")
     rng = np.random.Generator(np.random.PCG64(seed))
     nz, ny, nx = shape_zyx
     iz = torch.arange(nz, dtype=torch.float64, device=device)[:, None, None]
@@ -84,5 +89,5 @@ def gaussian_dvf(shape_zyx, seed: int, voxel_mm=1.0, bumps=8, device="cpu"):
         c = (rng.uniform(0, nx), rng.uniform(0, ny), rng.uniform(0, nz))
         dz += peak * torch.exp(-((ix - c[0]) ** 2 + (iy - c[1]) ** 2 + (iz - c[2]) ** 2) / (2 * sigma * sigma))
     dvf = torch.zeros((3, nz, ny, nx), dtype=torch.float32, device=device)
-    dvf[2] = dz.to(torch.float32)
+    dvf[2] = (dz / vz).to(torch.float32)
     return dvf

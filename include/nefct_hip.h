@@ -87,7 +87,7 @@ const char* nefct_status_string(int status);
 /* Shape limits of the Siddon entry points (NEFCT_ERR_SHAPE beyond them):
  * grid sides <= 16384, nv <= 65535, nb <= 262140, na * nb < 2^31 cells per view,
  * padded volume copies below 2^31 bytes.  The FDK backprojection takes
- * na, nb < 2^24 and na * nb < 2^29; the FDK filter na <= 2048. */
+ * na, nb < 2^23 and na * nb < 2^29; the FDK filter na <= 2048. */
 
 /* Bytes of device workspace the Siddon entry points need for this volume and
  * view count (padded volume copies + per-view table).  Replaces the

@@ -170,7 +170,13 @@ static void exact_walk(const orc_vol *V, const double S[3], const double D[3], o
 typedef struct { const double *img; double acc; } fwd_ctx;
 static void fwd_cb(void *p, long v, double l) { fwd_ctx *c = (fwd_ctx *)p; c->acc += c->img[v] * l; }
 typedef struct { double *img; double val; } bp_ctx;
-static void bp_cb(void *p, long v, double l) { bp_ctx *c = (bp_ctx *)p; c->img[v] += c->val * l; }
+static void bp_cb(void *p, long v, double l)
+{
+    bp_ctx *c = (bp_ctx *)p;
+    const double add = c->val * l;
+#pragma omp atomic
+    c->img[v] += add;
+}
 typedef struct { long count; double sum_len; } cnt_ctx;
 static void cnt_cb(void *p, long v, double l) { (void)v; cnt_ctx *c = (cnt_ctx *)p; c->count += 1; c->sum_len += l; }
 
@@ -211,14 +217,17 @@ void orc_exact_bp(const orc_vol *V, const orc_det *G, const double *angles, int 
 {
     const long nvox = (long)V->nx * V->ny * V->nz;
     memset(img, 0, sizeof(double) * nvox);
-    for (int iv = 0; iv < nv; ++iv)
-        for (int ib = 0; ib < G->nb; ++ib)
-            for (int ia = 0; ia < G->na; ++ia) {
-                double S[3], D[3];
-                orc_ray_points(G, angles[iv], ia, ib, S, D);
-                bp_ctx c = {img, pv[ia + (long)G->na * (ib + (long)G->nb * iv)]};
-                exact_walk(V, S, D, bp_cb, &c);
-            }
+    /* rays in parallel, each voxel update atomic: the summation order varies
+     * between runs at the level of FP64 rounding (~1e-16 relative) */
+    const long nrays = (long)nv * G->nb * G->na;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long r = 0; r < nrays; ++r) {
+        const int ia = (int)(r % G->na), ib = (int)((r / G->na) % G->nb), iv = (int)(r / ((long)G->na * G->nb));
+        double S[3], D[3];
+        orc_ray_points(G, angles[iv], ia, ib, S, D);
+        bp_ctx c = {img, pv[r]};
+        exact_walk(V, S, D, bp_cb, &c);
+    }
 }
 
 /* Per-ray intersection counts (voxels with nonzero length), and the total. */
