@@ -23,6 +23,7 @@
 #include <type_traits>
 #include <cmath>
 #include <initializer_list>
+#include <mutex>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -1877,6 +1878,75 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free adjoint: its scratch in a fixed device array
+// ---------------------------------------------------------------------------
+// The volume-tile adjoint needs, beyond its inputs and output, only the
+// max |projection| word of the fixed-point scale and the per-view (cos, sin)
+// table.  For callers that cannot allocate (graph capture, serving loops)
+// both live in `scratch`: bytes [0, 256) hold the max word, then 16 bytes per
+// view for at most NOALLOC_MAX_VIEWS views (128 KB).  Every device holds its
+// own copy until the process ends; in host code `scratch` names a host-side
+// stand-in, so its device address is resolved per device with
+// hipGetSymbolAddress and only that address is passed to kernels.  Calls that
+// overlap on one device share the array and race.
+constexpr int NOALLOC_MAX_VIEWS = 8192;
+constexpr size_t NOALLOC_TRIG_OFF = 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_TRIG_OFF + 2 * sizeof(double) * (size_t)NOALLOC_MAX_VIEWS;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[NOALLOC_MAX_DEVICES];   // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device, resolved on the first
+// call made with that device current.
+int scratch_of_current_device(char** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return NEFCT_ERR_LAUNCH;
+    if (dev >= NOALLOC_MAX_DEVICES) return NEFCT_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NEFCT_ERR_LAUNCH;
+        g_scratch_dev[dev] = (char*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return NEFCT_OK;
+}
+
+int run_adjoint_noalloc(const float* pv, const int grid[3], const float center[3], const float size[3],
+                        const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb,
+                        float db, float bi, int cyl, float* image, unsigned flags, void* stream)
+{
+    int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
+    if (st) return st;
+    if (!image || !pv) return NEFCT_ERR_NULL;
+    if (nv > NOALLOC_MAX_VIEWS) return NEFCT_ERR_SCRATCH;
+    if (flags & NEFCT_F_ADJ_WINDOW) return NEFCT_ERR_UNSUPPORTED;   // needs the padded-copy workspace
+    char* sc = nullptr;
+    st = scratch_of_current_device(&sc);
+    if (st) return st;
+    // the tile kernels read the view table and the max word only; the padded
+    // copies of the plan are never touched (their strides still describe the
+    // walk axes)
+    WsPlan p = plan_ws(grid, nv);
+    p.trig_off = NOALLOC_TRIG_OFF;
+    p.xf_off = p.yf_off = 0;
+    p.xf_elems = p.yf_elems = 0;
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, sc, p);
+    hipStream_t s = (hipStream_t)stream;
+    launch_trig(angles, nv, (double*)(sc + NOALLOC_TRIG_OFF), s);
+    launch_adjoint_tiles(L, pv, image, (unsigned*)sc, s);
+    NEFCT_CHECK_LAUNCH();
+    return NEFCT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* nefct_status_string(int status)
@@ -1889,6 +1959,7 @@ const char* nefct_status_string(int status)
     case NEFCT_ERR_WORKSPACE: return "workspace missing or too small";
     case NEFCT_ERR_UNSUPPORTED: return "unsupported flags or configuration";
     case NEFCT_ERR_LAUNCH: return "HIP kernel launch failed";
+    case NEFCT_ERR_SCRATCH: return "beyond the fixed scratch of the allocation-free entry point (at most 8192 views)";
     default: return "unknown status";
     }
 }
@@ -1924,6 +1995,15 @@ int nefct_backproject_flat(const float* projection_value, const int grid[3], con
 {
     return run_adjoint(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 0, image,
                        flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_backproject_flat_noalloc(const float* projection_value, const int grid[3], const float center[3],
+                                   const float size[3], const float* angles, int nv, float SID, float SAD, int na,
+                                   float da, float ai, int nb, float db, float bi, float* image, unsigned flags,
+                                   void* stream)
+{
+    return run_adjoint_noalloc(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 0,
+                               image, flags, stream);
 }
 
 int nefct_backproject_cyli(const float* projection_value, const int grid[3], const float center[3],

@@ -46,6 +46,7 @@ SIGNATURES = {
     "nefct_project_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_project_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_backproject_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_backproject_flat_noalloc": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp],
     "nefct_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_siddon_count": [_vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u, _vp, _sz, _vp],
     "nefct_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],

@@ -40,7 +40,8 @@ typedef enum {
     NEFCT_ERR_GEOMETRY = -3,    /* non-finite or degenerate geometry */
     NEFCT_ERR_WORKSPACE = -4,   /* workspace missing or too small */
     NEFCT_ERR_UNSUPPORTED = -5, /* flag combination not supported */
-    NEFCT_ERR_LAUNCH = -6       /* HIP launch failed */
+    NEFCT_ERR_LAUNCH = -6,      /* HIP launch failed */
+    NEFCT_ERR_SCRATCH = -7      /* beyond the fixed scratch of an allocation-free entry point */
 } nefct_status;
 
 /* Flags of the Siddon entry points.  0 = exact Siddon (the adjoint pair).
@@ -119,6 +120,21 @@ int nefct_backproject_flat(const float* projection_value, const int grid[3], con
                            const float size[3], const float* angles, int nv, float SID, float SAD,
                            int na, float da, float ai, int nb, float db, float bi, float* image,
                            unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Allocation-free variant of nefct_backproject_flat (no workspace argument).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Calls no hipMalloc/hipFree; its scratch (the
+ * fixed-point scale word and the per-view table) is a fixed 128 KB device
+ * array per GPU, held until the process ends.  Limits: nv <= 8192 views
+ * (NEFCT_ERR_SCRATCH beyond), and NEFCT_F_ADJ_WINDOW is refused
+ * (NEFCT_ERR_UNSUPPORTED: it needs the padded-copy workspace); every other
+ * flag and result is as nefct_backproject_flat.  Calls that overlap on one
+ * GPU share the scratch and race; calls on one stream are ordered. */
+int nefct_backproject_flat_noalloc(const float* projection_value, const int grid[3], const float center[3],
+                                   const float size[3], const float* angles, int nv, float SID, float SAD,
+                                   int na, float da, float ai, int nb, float db, float bi, float* image,
+                                   unsigned flags, void* stream);
 
 /* Adjoint, cylindrical detector.  Replaces BackProjectCyliOp::Compute
  * (siddon.cc:306-334) + backproject_cyli (siddon.cu.cc:863-890); reads the
