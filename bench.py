@@ -131,6 +131,135 @@ def batched_leg(dev, iters, log_every=True):
     }
 
 
+def c4_leg(dev, iters=3):
+    """4-D CT with deformation: ten respiratory phases of one 256x256x160 base
+    volume, each warped by its own Gaussian-bump displacement field and seen
+    from 40-120 views of a 384x288 detector.  One gradient pass = warp every
+    phase, project, backproject, transposed warps into the base gradient
+    (and the displacement gradients).  Plain: per-phase autograd calls in a
+    loop; batched: one warp per phase and one launch per direction for all
+    phases.  Also one graph-captured descent iteration on the base volume
+    (warps, batched pair, residual, transposed warps, update) replayed."""
+    import torch
+    import nefct_hip as nh
+    from nefct_hip.batch import SiddonBatch
+    from nefct_hip.configs import c4
+    from nefct_hip.deform import deform_adjoint_op, deform_op, warp
+    from nefct_hip.fourd import warp_project
+    from nefct_hip.phantom import ellipsoid_phantom, gaussian_dvf
+
+    phases = c4()
+    vol = phases[0].vol
+    log(f"[bench] C4 leg: {len(phases)} phases, {sum(len(p.angles) for p in phases)} views")
+    base = ellipsoid_phantom(vol, phases[0].phantom_seed, 10, device=dev)
+    dvfs = [gaussian_dvf(vol.shape_zyx, p.dvf_seed, voxel_mm=vol.voxel, device=dev) for p in phases]
+    angs = [torch.tensor(p.angles, device=dev) for p in phases]
+    batch = SiddonBatch([(p.vol, p.det, p.angles) for p in phases], device=dev)
+    g = torch.Generator(device=dev).manual_seed(56)
+    ys = [torch.randn(s.shape, generator=g, device=dev) for s in batch.sinograms]
+    rays = batch.rays
+
+    def plain():
+        x = base.detach().requires_grad_(True)
+        ds = [d.detach().requires_grad_(True) for d in dvfs]
+        loss = 0.0
+        for p, d, a, y in zip(phases, ds, angs, ys):
+            loss = loss + (nh.forward_project(warp(x, d, 1), p.vol, p.det, a) * y).sum()
+        loss.backward()
+
+    def batched():
+        x = base.detach().requires_grad_(True)
+        ds = [d.detach().requires_grad_(True) for d in dvfs]
+        outs = warp_project(batch, [x] * len(phases), ds, boundary=1)
+        sum((o * y).sum() for o, y in zip(outs, ys)).backward()
+
+    res = {}
+    for name, fn in (("plain_warp_pair", plain), ("batched_warp_pair", batched)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / iters
+        res[name] = (rays / dt / 1e9, dt)
+        log(f"[bench] C4 leg {name}: {dt * 1e3:.1f} ms per pass, {res[name][0]:.4f} Grays/s")
+
+    # one descent iteration on the base volume, captured in a graph
+    x = (0.5 * base).contiguous()
+    grad = torch.zeros_like(base)
+    targets = [t.clone() for t in ys]
+
+    def iteration():
+        for p, d in enumerate(dvfs):
+            batch.volumes[p].copy_(deform_op(x, d[0], d[1], d[2], 1))
+        batch.project()
+        for s_, t_ in zip(batch.sinograms, targets):
+            s_.sub_(t_)
+        batch.backproject()
+        grad.zero_()
+        for p, d in enumerate(dvfs):
+            grad.add_(deform_adjoint_op(batch.volumes[p], d[0], d[1], d[2], 1))
+        x.add_(grad, alpha=-1e-6)
+
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        iteration()
+    torch.cuda.current_stream().wait_stream(st)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        iteration()
+    graph.replay()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    t_graph = e0.elapsed_time(e1) / 1e3 / iters
+    log(f"[bench] C4 leg graph-captured iteration: {t_graph * 1e3:.1f} ms")
+    return {
+        "workload": "C4: 10 phases of a 256x256x160 @ 1 mm base, own Gaussian-bump warp each, 40-120 views, "
+                    "384x288 detector; gradient pass to base volume and warp fields",
+        "rays": rays,
+        "unit": "Grays/s",
+        "plain_warp_pair": round(res["plain_warp_pair"][0], 5),
+        "batched_warp_pair": round(res["batched_warp_pair"][0], 5),
+        "speedup_warp_pair": round(res["batched_warp_pair"][0] / res["plain_warp_pair"][0], 3),
+        "ms_plain": round(res["plain_warp_pair"][1] * 1e3, 2),
+        "ms_batched": round(res["batched_warp_pair"][1] * 1e3, 2),
+        "graph_iteration_ms": round(t_graph * 1e3, 2),
+        "graph_iteration_grays_s": round(rays / t_graph / 1e9, 5),
+        "iters": iters,
+    }
+
+
+def copy_bandwidth(dev, nbytes=2 << 30, reps=5):
+    """Achievable streaming bandwidth: a device-to-device copy of nbytes
+    (read nbytes + write nbytes per copy), HIP-event timed over reps copies."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    gbs = 2 * nbytes / t / 1e9
+    return {"bytes_moved": 2 * nbytes, "ms": round(t * 1e3, 3), "GBs": round(gbs, 1),
+            "pct_of_spec": round(100 * gbs / HBM_PEAK_GBS, 1)}
+
+
 def _timed(fn, reps):
     """Mean HIP-event time (s) of fn() over reps calls on torch's stream, after one untimed call."""
     import torch
@@ -226,7 +355,9 @@ def main():
                     help="profiling only: adjoint algorithm flags (16 deterministic, 32 slab windows)")
     ap.add_argument("--only", choices=["pair", "fwd", "adj"], default="pair",
                     help="profiling only: run one direction")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default launch shapes instead of the autotuner's (tuning is untimed)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain and C4 legs")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain leg")
     ap.add_argument("--batched-iters", type=int, default=2)
     ap.add_argument("--no-fdk", action="store_true", help="skip the FDK and warp legs")
@@ -272,14 +403,28 @@ def main():
     bytes_fwd = 4 * n_int + 4 * rays
     bytes_adj = 4 * rays + 4 * n_int
 
+    # launch shapes: the autotuner's choice for this geometry (timed on a view
+    # sample before the timed region), unless fixed on the command line
+    fwd_flags, adj_flags = 0, args.adj_flags
+    tuned = None
+    if not args.no_tune and not args.adj_flags:
+        from nefct_hip import autotune
+        log("[bench] autotuning the forward and adjoint launch shapes")
+        fwd_flags, frec = autotune.tune_forward(vol, det, ang)
+        adj_flags, arec = autotune.tune_adjoint(vol, det, ang)
+        tuned = {"forward": frec["variant"], "adjoint": arec["variant"],
+                 "forward_ms_on_sample": frec["ms_on_sample"], "adjoint_ms_on_sample": arec["ms_on_sample"],
+                 "forward_sample_views": frec["sample_views"], "adjoint_sample_views": arec["sample_views"]}
+        log(f"[bench] tuned: forward {frec['variant']}, adjoint {arec['variant']}")
+
     def step():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e2 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        sino = nh.project(img, vol, det, ang) if args.only != "adj" else sino_fixed
+        sino = nh.project(img, vol, det, ang, flags=fwd_flags) if args.only != "adj" else sino_fixed
         e1.record()
-        back = nh.backproject(sino, vol, det, ang, flags=args.adj_flags) if args.only != "fwd" else None
+        back = nh.backproject(sino, vol, det, ang, flags=adj_flags) if args.only != "fwd" else None
         e2.record()
         return e0, e1, e2, sino, back
 
@@ -318,11 +463,13 @@ def main():
         warp_rates = warp_leg(wl, img)
         torch.cuda.empty_cache()
 
-    batched = None
+    batched = c4 = None
     if not args.no_batched and not args.views and args.only == "pair":
         del sino_fixed
         torch.cuda.empty_cache()
         batched = batched_leg(dev, args.batched_iters)
+        torch.cuda.empty_cache()
+        c4 = c4_leg(dev)
 
     cpu = None
     if not args.no_cpu:
@@ -330,15 +477,25 @@ def main():
         cpu = cpu_baseline(wl, img.cpu(), args.cpu_stride)
         log(f"[bench] CPU baseline {cpu['value']:.4f} Grays/s on {cpu['cores']} threads")
 
-    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
-    # FETCH_SIZE / WRITE_SIZE (scaled by a known-size copy) and committed
+    # HBM bytes per launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (scaled by a known-size copy) and committed; with this run's kernel
+    # times they give each kernel's measured share of the HBM peak
+    copy_bw = copy_bandwidth(dev) if not args.views else None
     traffic = None
+    measured = {}
     tpath = os.path.join(ROOT, "profiles", "siddon_pair_c2_traffic.json")
     if not args.views and os.path.exists(tpath):
         with open(tpath) as f:
-            tk = json.load(f)["kernels"].get(dominant)
-        if tk:
-            traffic = tk["traffic_bytes"]
+            tks = json.load(f)["kernels"]
+        for name, t_k in (("siddon_forward", t_fwd), ("siddon_adjoint", t_adj)):
+            tk = tks.get(name)
+            if tk:
+                gbs = tk["traffic_bytes"] / t_k / 1e9
+                measured[name] = {"traffic_bytes": tk["traffic_bytes"], "GBs": round(gbs, 1),
+                                  "frac_of_spec": round(gbs / HBM_PEAK_GBS, 4),
+                                  "frac_of_copy": round(gbs / copy_bw["GBs"], 4) if copy_bw else None}
+        if dominant in measured:
+            traffic = measured[dominant]["traffic_bytes"]
 
     rec = {
         "metric": "Siddon fwd+adjoint Grays/s at 512x512x256, 720 views",
@@ -359,6 +516,8 @@ def main():
             "detector": [det.na, det.nb], "pitch_mm": det.da, "views": int(ang.numel()),
             "sod_mm": det.SID, "sdd_mm": det.SAD, "rays": rays, "sum_n_int": n_int,
             "parallelism": "single GPU",
+            "launch_shapes": tuned or {"forward": "16x16/auto", "adjoint": "default" if not args.adj_flags
+                                       else f"flags {args.adj_flags}"},
         },
         "fwd_grays_s": round(rays / t_fwd / 1e9, 5),
         "adj_grays_s": round(rays / t_adj / 1e9, 5),
@@ -376,8 +535,15 @@ def main():
             if traffic else None,
             "algorithmic_bytes": b_dom,
             "avg_launch_ms": round(t_dom * 1e3, 3),
+            "frac_of_copy": round(achieved / copy_bw["GBs"], 4) if copy_bw else None,
+            "note": "achieved counts every voxel gather as HBM bytes; most are served by L2 "
+                    "(the forward exceeds the HBM peak by this measure), so hbm_measured gives the real "
+                    "HBM traffic; both Siddon kernels are bound by instruction issue",
         },
+        "hbm_measured": measured or None,
+        "copy_bandwidth": copy_bw,
         "batched": batched,
+        "c4": c4,
         "fdk": fdk,
         "warp": warp_rates,
         "cpu_baseline": cpu,

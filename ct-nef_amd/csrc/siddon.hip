@@ -129,7 +129,7 @@ __device__ __forceinline__ int ceil_clamp(float v, int lo, int hi)
 }
 
 __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, int ia, int ib,
-                                           double S[3], double D[3])
+                                           double S[3], double D[3], double dz = 0.0)
 {
     const double a = (g.ai + ia) * g.da - g.na * g.da * 0.5;
     const double b = (g.bi + ib) * g.db - g.nb * g.db * 0.5;
@@ -143,10 +143,10 @@ __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, 
     }
     S[0] = -g.sid * c;
     S[1] = -g.sid * s;
-    S[2] = 0.0;
+    S[2] = dz;
     D[0] = lx * c - ly * s;
     D[1] = lx * s + ly * c;
-    D[2] = b;
+    D[2] = b + dz;
 }
 
 // Restricts [tin, tout] to the slab parameters where lo <= A + k t <= hi
@@ -174,9 +174,10 @@ struct AxisGeom {
 };
 
 // The view's source in voxel units (the S of ray_points).
-__device__ __forceinline__ void source_voxels(const VolDev& V, const DetDev& G, double c, double s, double uS[3])
+__device__ __forceinline__ void source_voxels(const VolDev& V, const DetDev& G, double c, double s, double uS[3],
+                                              double dz = 0.0)
 {
-    const double S[3] = {-G.sid * c, -G.sid * s, 0.0};
+    const double S[3] = {-G.sid * c, -G.sid * s, dz};
     for (int i = 0; i < 3; ++i) uS[i] = (S[i] - V.lo[i]) * V.inv_vs[i];
 }
 
@@ -186,10 +187,10 @@ __device__ __forceinline__ void source_voxels(const VolDev& V, const DetDev& G, 
 // table), with the same values either way.
 template <class Geom>
 __device__ __forceinline__ void setup_ray_g(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
-                                            unsigned flags, bool adjoint, Geom&& geom, Ray& r)
+                                            unsigned flags, bool adjoint, Geom&& geom, Ray& r, double dz = 0.0)
 {
     double S[3], D[3], d[3];
-    ray_points(G, c, s, ia, ib, S, D);
+    ray_points(G, c, s, ia, ib, S, D, dz);
     for (int i = 0; i < 3; ++i) d[i] = (D[i] - S[i]) * V.inv_vs[i];
     r.dom = -1;
     r.s0 = r.s1 = 0;
@@ -270,11 +271,11 @@ __device__ __forceinline__ AxisGeom axis_geom(const VolDev& V, const double uS[3
 }
 
 __device__ __forceinline__ void setup_ray(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
-                                          unsigned flags, bool adjoint, Ray& r)
+                                          unsigned flags, bool adjoint, Ray& r, double dz = 0.0)
 {
     double uS[3];
-    source_voxels(V, G, c, s, uS);
-    setup_ray_g(V, G, c, s, ia, ib, flags, adjoint, [&](int m) { return axis_geom(V, uS, m); }, r);
+    source_voxels(V, G, c, s, uS, dz);
+    setup_ray_g(V, G, c, s, ia, ib, flags, adjoint, [&](int m) { return axis_geom(V, uS, m); }, r, dz);
 }
 
 __device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts& Ly)
@@ -491,10 +492,20 @@ struct Launch {
     DetDev G;
     Layouts Ly;
     const double* trig;  // (cos, sin) per view
+    const float* zoff;   // per-view z offset of source and detector (mm), or null
     int nv;
     unsigned flags;
     float sbound;        // upper bound of any ray's mm per slab (fixed-point adjoint scale)
 };
+
+// A helical view's lift: source and detector of view iv raised by zoff[iv]
+// along z (dist_3d_back_fdk.cu.cc:46,65); 0 without offsets.  (Passed to the
+// set-up as a value: a per-view copy of the volume record would be indexed
+// by axis at run time and so be placed in scratch memory.)
+__device__ __forceinline__ double view_lift(const Launch& P, int iv)
+{
+    return P.zoff ? (double)P.zoff[iv] : 0.0;
+}
 
 // Upper bound of Ray::scale over every ray of the geometry (mm of ray per
 // slab, times 1/L^2 under INV_L2), with 1 % margin.  Exact mode: the walk axis
@@ -571,7 +582,7 @@ __device__ __forceinline__ void siddon_rays(const Launch& P, int ia, int ib, int
     r.s0 = r.s1 = r.tc = 0;
     r.ka = r.kb = r.Ac = r.Bc = r.inv_ka = r.inv_kb = r.pa = r.pb = 0.0f;
     r.scale = 0.0f;
-    if (inside) setup_ray(P.V, P.G, c, s, ia, ib, P.flags, MODE == 1, r);
+    if (inside) setup_ray(P.V, P.G, c, s, ia, ib, P.flags, MODE == 1, r, view_lift(P, iv));
     const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
     float val = 0.0f;
     if (MODE == 1 && inside) val = pv_in[id];
@@ -598,8 +609,11 @@ __host__ inline int fwd_group_log2(int nv)
     return g;
 }
 
-template <int MODE, bool REF>
-__global__ __launch_bounds__(FX * FY) void k_siddon(Launch P, const float* __restrict__ pv_in,
+// TX x TY rays per workgroup (256 threads, x fastest: a wavefront is TX x
+// 64/TX rays); 16 x 16 by default, the others selectable by the forward tile
+// flags for the autotuner.
+template <int MODE, bool REF, int TX = FX, int TY = FY>
+__global__ __launch_bounds__(TX * TY) void k_siddon(Launch P, const float* __restrict__ pv_in,
                                                      float* __restrict__ pv_out, int32_t* __restrict__ counts,
                                                      int tiles_a, int tiles_b, int v0, int nvc, int vg_log2)
 {
@@ -611,27 +625,49 @@ __global__ __launch_bounds__(FX * FY) void k_siddon(Launch P, const float* __res
     const int iv = (int)((group << vg_log2) + vin);
     if (iv >= nvc) return;
     const int ta = (int)(tile % (unsigned)tiles_a), tb = (int)(tile / (unsigned)tiles_a);
-    siddon_rays<MODE, REF>(P, ta * FX + threadIdx.x, tb * FY + threadIdx.y, v0 + iv, pv_in, pv_out, counts);
+    siddon_rays<MODE, REF>(P, ta * TX + threadIdx.x, tb * TY + threadIdx.y, v0 + iv, pv_in, pv_out, counts);
+}
+
+// View group of the forward's work order: log2 of the views an XCD walks
+// together, from the flags (NEFCT_F_FWD_VG_*) or the default rule.
+__host__ inline int fwd_vg_log2(unsigned flags, int nv)
+{
+    const unsigned v = (flags & NEFCT_F_FWD_VG_MASK) >> NEFCT_F_FWD_VG_SHIFT;
+    return v ? (int)v : fwd_group_log2(nv);
 }
 
 // Launches k_siddon over all views in chunks whose 1-D grids stay below 2^31
 // work-items.
-template <int MODE, bool REF>
-void launch_siddon(const Launch& L, const float* pv_in, float* pv_out, int32_t* counts, hipStream_t s)
+template <int MODE, bool REF, int TX, int TY>
+void launch_siddon_tiles(const Launch& L, const float* pv_in, float* pv_out, int32_t* counts, hipStream_t s)
 {
-    const int ta = (L.G.na + FX - 1) / FX, tb = (L.G.nb + FY - 1) / FY;
+    const int ta = (L.G.na + TX - 1) / TX, tb = (L.G.nb + TY - 1) / TY;
     const long long tiles = (long long)ta * tb;
-    const int vgl = fwd_group_log2(L.nv), vg = 1 << vgl;
-    long long max_groups = (0x7fffffffll / (FX * FY)) / (tiles * vg);
+    const int vgl = fwd_vg_log2(L.flags, L.nv), vg = 1 << vgl;
+    long long max_groups = (0x7fffffffll / (TX * TY)) / (tiles * vg);
     if (max_groups < 1) max_groups = 1;
     const int chunk = (int)std::min<long long>(L.nv, max_groups * vg);
     for (int v0 = 0; v0 < L.nv; v0 += chunk) {
         const int nvc = std::min(chunk, L.nv - v0);
         const long long groups = (nvc + vg - 1) / vg;
         const long long blocks = (tiles * groups * vg + 7) / 8 * 8;
-        hipLaunchKernelGGL((k_siddon<MODE, REF>), dim3((unsigned)blocks), dim3(FX, FY), 0, s, L, pv_in, pv_out,
-                           counts, ta, tb, v0, nvc, vgl);
+        hipLaunchKernelGGL((k_siddon<MODE, REF, TX, TY>), dim3((unsigned)blocks), dim3(TX, TY), 0, s, L, pv_in,
+                           pv_out, counts, ta, tb, v0, nvc, vgl);
     }
+}
+
+template <int MODE, bool REF>
+void launch_siddon(const Launch& L, const float* pv_in, float* pv_out, int32_t* counts, hipStream_t s)
+{
+    const unsigned t = (MODE == 0) ? (L.flags & NEFCT_F_FWD_TILE_MASK) : NEFCT_F_FWD_TILE_16x16;
+    if (t == NEFCT_F_FWD_TILE_32x8)
+        launch_siddon_tiles<MODE, REF, 32, 8>(L, pv_in, pv_out, counts, s);
+    else if (t == NEFCT_F_FWD_TILE_8x32)
+        launch_siddon_tiles<MODE, REF, 8, 32>(L, pv_in, pv_out, counts, s);
+    else if (t == NEFCT_F_FWD_TILE_64x4)
+        launch_siddon_tiles<MODE, REF, 64, 4>(L, pv_in, pv_out, counts, s);
+    else
+        launch_siddon_tiles<MODE, REF, FX, FY>(L, pv_in, pv_out, counts, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -774,6 +810,7 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
     const int ia = blockIdx.x * BX + lane;
     const int iv = blockIdx.z;
     const double c = P.trig[2 * iv], s = P.trig[2 * iv + 1];
+    const double dz = view_lift(P, iv);
     Ray r[AR];
     bool in[AR];
     float sv[AR];
@@ -787,7 +824,7 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
         r[j].scale = 0.0f;
         sv[j] = 0.0f;
         if (in[j]) {
-            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r[j]);
+            setup_ray(P.V, P.G, c, s, ia, ib, P.flags, true, r[j], dz);
             const size_t id = (size_t)ia + (size_t)P.G.na * ((size_t)ib + (size_t)P.G.nb * (size_t)iv);
             sv[j] = pv_in[id] * r[j].scale;
         }
@@ -960,7 +997,7 @@ __device__ __forceinline__ void lds_deposit_s(char* base, int byte_off, int q)
 // as (ia0, ia1, ib0, ib1) (empty when ia1 < ia0).  Computed by 8 lanes
 // together: lane q (0..7) of the group takes corner q.
 __device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, double c, double s, const int lo[3],
-                                              const int hi[3], bool tight, int q)
+                                              const int hi[3], bool tight, int q, double dz = 0.0)
 {
     // Fractional detector indices of the 8 corners (flat: a = t SDD / U,
     // b = z SDD / U; cylindrical: a = atan2(t, U), b = z SDD / hypot(U, t)),
@@ -968,7 +1005,7 @@ __device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, 
     // every lane of the group ends with the same bounds.
     const double x = V.lo[0] + V.vs[0] * ((q & 1) ? hi[0] : lo[0]);
     const double y = V.lo[1] + V.vs[1] * ((q & 2) ? hi[1] : lo[1]);
-    const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]);
+    const double z = V.lo[2] + V.vs[2] * ((q & 4) ? hi[2] : lo[2]) - dz;   // relative to the lifted source
     const double U = G.sid + x * c + y * s;
     const double t = -x * s + y * c;
     int behind = (U <= 1e-6 * G.sid) ? 1 : 0;
@@ -1087,10 +1124,11 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
             for (int e = tid >> 3; e < n; e += 8 * NW) {   // the 8 lanes of a group share e
                 const int jv = iv + e;
                 const double cj = P.trig[2 * jv], sj = P.trig[2 * jv + 1];
-                const int4 sh = shadow_of_box(P.V, P.G, cj, sj, b0, b1, !REF, tid & 7);
+                const double dzj = view_lift(P, jv);
+                const int4 sh = shadow_of_box(P.V, P.G, cj, sj, b0, b1, !REF, tid & 7, dzj);
                 if ((tid & 7) == 0) {
                     shadows[e] = sh;
-                    source_voxels(P.V, P.G, cj, sj, src_tab[e]);
+                    source_voxels(P.V, P.G, cj, sj, src_tab[e], dzj);
                 }
             }
             __syncthreads();
@@ -1546,8 +1584,10 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0)) return NEFCT_ERR_GEOMETRY;
     if (!(SAD > 0)) return NEFCT_ERR_GEOMETRY;
     if (flags & ~(NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT | NEFCT_F_DETERMINISTIC |
-                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_FLOAT_ATOMIC | NEFCT_F_TILE_MASK | NEFCT_F_LANE_STRIDE2))
+                  NEFCT_F_ADJ_WINDOW | NEFCT_F_ADJ_FLOAT_ATOMIC | NEFCT_F_TILE_MASK | NEFCT_F_LANE_STRIDE2 |
+                  NEFCT_F_FWD_TILE_MASK | NEFCT_F_FWD_VG_MASK))
         return NEFCT_ERR_UNSUPPORTED;
+    if (((flags & NEFCT_F_FWD_VG_MASK) >> NEFCT_F_FWD_VG_SHIFT) > 6) return NEFCT_ERR_UNSUPPORTED;
     if ((flags & NEFCT_F_TILE_MASK) > NEFCT_F_TILE_64x16x16) return NEFCT_ERR_UNSUPPORTED;
     if ((flags & NEFCT_F_DETERMINISTIC) && (flags & NEFCT_F_ADJ_WINDOW)) return NEFCT_ERR_UNSUPPORTED;
     if (flags & NEFCT_F_REF_SLAB) {
@@ -1560,9 +1600,10 @@ int validate(const int grid[3], const float center[3], const float size[3], cons
 
 Launch make_launch(const int grid[3], const float center[3], const float size[3], int nv, float SID, float SAD,
                    int na, float da, float ai, int nb, float db, float bi, int cyl, unsigned flags, char* ws,
-                   const WsPlan& p)
+                   const WsPlan& p, const float* zoff = nullptr)
 {
     Launch L;
+    L.zoff = zoff;
     for (int i = 0; i < 3; ++i) {
         L.V.n[i] = grid[i];
         L.V.vs[i] = (double)size[i] / grid[i];
@@ -1617,7 +1658,8 @@ void launch_zero(float* ptr, size_t n, hipStream_t st)
 
 int run_forward(const float* image, const int grid[3], const float center[3], const float size[3],
                 const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb, float db,
-                float bi, int cyl, float* out, unsigned flags, void* workspace, size_t wsbytes, void* stream)
+                float bi, int cyl, float* out, unsigned flags, void* workspace, size_t wsbytes, void* stream,
+                const float* zoff = nullptr)
 {
     int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
     if (st) return st;
@@ -1626,7 +1668,7 @@ int run_forward(const float* image, const int grid[3], const float center[3], co
     if (!workspace || wsbytes < p.total) return NEFCT_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
-    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p, zoff);
     launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
     launch_pad(image, grid, p, ws, s);
     if (flags & NEFCT_F_REF_SLAB)
@@ -1641,7 +1683,8 @@ void launch_adjoint_tiles(const Launch& L, const float* pv, float* image, unsign
 
 int run_adjoint(const float* pv, const int grid[3], const float center[3], const float size[3], const float* angles,
                 int nv, float SID, float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
-                float* image, unsigned flags, void* workspace, size_t wsbytes, void* stream)
+                float* image, unsigned flags, void* workspace, size_t wsbytes, void* stream,
+                const float* zoff = nullptr)
 {
     int st = validate(grid, center, size, angles, nv, SID, SAD, na, da, nb, db, ai, bi, flags);
     if (st) return st;
@@ -1650,7 +1693,7 @@ int run_adjoint(const float* pv, const int grid[3], const float center[3], const
     if (!workspace || wsbytes < p.total) return NEFCT_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)workspace;
-    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p);
+    Launch L = make_launch(grid, center, size, nv, SID, SAD, na, da, ai, nb, db, bi, cyl, flags, ws, p, zoff);
     launch_trig(angles, nv, (double*)(ws + p.trig_off), s);
     const int nx = grid[0], ny = grid[1], nz = grid[2];
     const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
@@ -1727,6 +1770,7 @@ constexpr int Q_HEADER = 256;            // workspace head (reserved), then one 
 __device__ Launch launch_of(const nefct_problem& pr, char* ws_base, unsigned flags)
 {
     Launch L;
+    L.zoff = nullptr;
     for (int i = 0; i < 3; ++i) {
         L.V.n[i] = pr.grid[i];
         L.V.vs[i] = (double)pr.size[i] / pr.grid[i];
@@ -2013,6 +2057,26 @@ int nefct_backproject_cyli(const float* projection_value, const int grid[3], con
 {
     return run_adjoint(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, 1, image,
                        flags, workspace, workspace_bytes, stream);
+}
+
+int nefct_project_helical(const float* image, const int grid[3], const float center[3], const float size[3],
+                          const float* angles, const float* offsets, int nv, float SID, float SAD, int na, float da,
+                          float ai, int nb, float db, float bi, int cyl, float* projection_value, unsigned flags,
+                          void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!offsets) return NEFCT_ERR_NULL;
+    return run_forward(image, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi, cyl ? 1 : 0,
+                       projection_value, flags, workspace, workspace_bytes, stream, offsets);
+}
+
+int nefct_backproject_helical(const float* projection_value, const int grid[3], const float center[3],
+                              const float size[3], const float* angles, const float* offsets, int nv, float SID,
+                              float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
+                              float* image, unsigned flags, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!offsets) return NEFCT_ERR_NULL;
+    return run_adjoint(projection_value, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi,
+                       cyl ? 1 : 0, image, flags, workspace, workspace_bytes, stream, offsets);
 }
 
 int nefct_siddon_count(const int grid[3], const float center[3], const float size[3], const float* angles, int nv,

@@ -28,6 +28,20 @@ F_TILE_64x32x8 = 0x300
 F_TILE_16x16x32 = 0x400
 F_TILE_64x16x16 = 0x500
 F_LANE_STRIDE2 = 0x800
+F_FWD_TILE_MASK = 0x6000
+F_FWD_TILE_16x16 = 0x0000
+F_FWD_TILE_32x8 = 0x2000
+F_FWD_TILE_8x32 = 0x4000
+F_FWD_TILE_64x4 = 0x6000
+F_FWD_VG_SHIFT = 15
+F_FWD_VG_MASK = 7 << F_FWD_VG_SHIFT
+
+
+def fwd_vg(log2_views: int) -> int:
+    """Forward view-group field for 2^log2_views views (1..6); 0 = default rule."""
+    if not 0 <= log2_views <= 6:
+        raise ValueError("view group log2 must be in 0..6")
+    return log2_views << F_FWD_VG_SHIFT
 REF_FORWARD = F_INV_L2 | F_REF_SLAB
 REF_ADJOINT = F_REF_SLAB | F_SHORT_SKIP | F_TIE_GT
 
@@ -48,6 +62,10 @@ SIGNATURES = {
     "nefct_backproject_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
     "nefct_backproject_flat_noalloc": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp],
     "nefct_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _u, _vp, _sz, _vp],
+    "nefct_project_helical": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u, _vp,
+                              _sz, _vp],
+    "nefct_backproject_helical": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u,
+                                  _vp, _sz, _vp],
     "nefct_siddon_count": [_vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _vp, _u, _vp, _sz, _vp],
     "nefct_deform": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nefct_deform_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],

@@ -25,20 +25,29 @@ def _angles(angles, device):
     return angles.to(device=device, dtype=torch.float32).contiguous()
 
 
-def project(image: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, flags: int = 0) -> torch.Tensor:
-    """Line integrals [nv, nb, na] of an x-fastest volume [nz, ny, nx]."""
+def _offsets(offsets, device):
+    if offsets is None:
+        return None
+    return _angles(offsets, device)
+
+
+def project(image: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, flags: int = 0,
+            offsets=None) -> torch.Tensor:
+    """Line integrals [nv, nb, na] of an x-fastest volume [nz, ny, nx].
+    ``offsets`` (mm, one per view): helical lift of source and detector along z."""
     g, c, s = _args(vol, det)
     a = _angles(angles, image.device)
     return ops.siddon_project(image, g, c, s, a, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db,
-                              det.bi, det.cyl, int(flags))
+                              det.bi, det.cyl, int(flags), _offsets(offsets, image.device))
 
 
-def backproject(sino: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, flags: int = 0) -> torch.Tensor:
+def backproject(sino: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles, flags: int = 0,
+                offsets=None) -> torch.Tensor:
     """Adjoint of ``project`` (exact transpose for flags 0): [nv, nb, na] -> [nz, ny, nx]."""
     g, c, s = _args(vol, det)
     a = _angles(angles, sino.device)
     return ops.siddon_backproject(sino, g, c, s, a, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db,
-                                  det.bi, det.cyl, int(flags))
+                                  det.bi, det.cyl, int(flags), _offsets(offsets, sino.device))
 
 
 def count_intersections(vol: VolumeGeometry, det: ConeGeometry, angles, device="cuda") -> torch.Tensor:
@@ -54,41 +63,41 @@ class SiddonProjection(torch.autograd.Function):
     hand-written adjoint kernel."""
 
     @staticmethod
-    def forward(ctx, image, vol, det, angles):
-        ctx.geom = (vol, det)
+    def forward(ctx, image, vol, det, angles, offsets=None):
+        ctx.geom = (vol, det, offsets)
         ctx.save_for_backward(angles)
-        return project(image, vol, det, angles, 0)
+        return project(image, vol, det, angles, 0, offsets)
 
     @staticmethod
     def backward(ctx, grad):
         (angles,) = ctx.saved_tensors
-        vol, det = ctx.geom
-        gi = backproject(grad.contiguous(), vol, det, angles, 0) if ctx.needs_input_grad[0] else None
-        return gi, None, None, None
+        vol, det, offsets = ctx.geom
+        gi = backproject(grad.contiguous(), vol, det, angles, 0, offsets) if ctx.needs_input_grad[0] else None
+        return gi, None, None, None, None
 
 
 class SiddonBackprojection(torch.autograd.Function):
     """x = A^T y; backward applies A."""
 
     @staticmethod
-    def forward(ctx, sino, vol, det, angles):
-        ctx.geom = (vol, det)
+    def forward(ctx, sino, vol, det, angles, offsets=None):
+        ctx.geom = (vol, det, offsets)
         ctx.save_for_backward(angles)
-        return backproject(sino, vol, det, angles, 0)
+        return backproject(sino, vol, det, angles, 0, offsets)
 
     @staticmethod
     def backward(ctx, grad):
         (angles,) = ctx.saved_tensors
-        vol, det = ctx.geom
-        gs = project(grad.contiguous(), vol, det, angles, 0) if ctx.needs_input_grad[0] else None
-        return gs, None, None, None
+        vol, det, offsets = ctx.geom
+        gs = project(grad.contiguous(), vol, det, angles, 0, offsets) if ctx.needs_input_grad[0] else None
+        return gs, None, None, None, None
 
 
-def forward_project(image, vol, det, angles):
+def forward_project(image, vol, det, angles, offsets=None):
     a = _angles(angles, image.device)
-    return SiddonProjection.apply(image, vol, det, a)
+    return SiddonProjection.apply(image, vol, det, a, _offsets(offsets, image.device))
 
 
-def back_project(sino, vol, det, angles):
+def back_project(sino, vol, det, angles, offsets=None):
     a = _angles(angles, sino.device)
-    return SiddonBackprojection.apply(sino, vol, det, a)
+    return SiddonBackprojection.apply(sino, vol, det, a, _offsets(offsets, sino.device))

@@ -79,6 +79,18 @@ typedef enum {
 #define NEFCT_F_TILE_16x16x32 0x400u
 #define NEFCT_F_TILE_64x16x16 0x500u
 #define NEFCT_F_LANE_STRIDE2 0x800u  /* neighbouring lanes take every other detector row */
+/* Forward work mapping (ignored by the adjoint; same result): rays per
+ * workgroup (columns x rows, 256 threads; a wavefront is the first 64), and
+ * the number of views an XCD walks together (2^k views for k = 1..6 in the
+ * NEFCT_F_FWD_VG field; 0 = chosen from the view count, about 16 degrees of
+ * a full scan).  The autotuner picks among these. */
+#define NEFCT_F_FWD_TILE_MASK 0x6000u
+#define NEFCT_F_FWD_TILE_16x16 0x0000u  /* default */
+#define NEFCT_F_FWD_TILE_32x8 0x2000u
+#define NEFCT_F_FWD_TILE_8x32 0x4000u
+#define NEFCT_F_FWD_TILE_64x4 0x6000u
+#define NEFCT_F_FWD_VG_SHIFT 15
+#define NEFCT_F_FWD_VG_MASK (7u << NEFCT_F_FWD_VG_SHIFT)
 #define NEFCT_REF_FORWARD (NEFCT_F_INV_L2 | NEFCT_F_REF_SLAB)
 #define NEFCT_REF_ADJOINT (NEFCT_F_REF_SLAB | NEFCT_F_SHORT_SKIP | NEFCT_F_TIE_GT)
 
@@ -144,6 +156,22 @@ int nefct_backproject_cyli(const float* projection_value, const int grid[3], con
                            const float size[3], const float* angles, int nv, float SID, float SAD,
                            int na, float da, float ai, int nb, float db, float bi, float* image,
                            unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Helical forward projection and adjoint: as nefct_project_flat/_cyli and
+ * nefct_backproject_flat/_cyli (cyl = 0 flat, 1 cylindrical), with view iv's
+ * source and detector lifted by offsets[iv] mm along z (device array of nv
+ * floats, required).  Replaces the per-view 'offsets' the reference's Python
+ * callers pass to their projector (nefct/functions/project.py:74-76,
+ * back_project.py:157, project_t.py:157), with the lift of
+ * dist_3d_back_fdk.cu.cc:46,65.  The pair stays an exact transpose. */
+int nefct_project_helical(const float* image, const int grid[3], const float center[3], const float size[3],
+                          const float* angles, const float* offsets, int nv, float SID, float SAD, int na,
+                          float da, float ai, int nb, float db, float bi, int cyl, float* projection_value,
+                          unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
+int nefct_backproject_helical(const float* projection_value, const int grid[3], const float center[3],
+                              const float size[3], const float* angles, const float* offsets, int nv, float SID,
+                              float SAD, int na, float da, float ai, int nb, float db, float bi, int cyl,
+                              float* image, unsigned flags, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Per-ray count of voxels with nonzero length (N_int) as walked by the exact
  * kernels; counts {nv, nb, na} int32.  No reference counterpart: it is the

@@ -27,6 +27,46 @@ def test_candidates_are_valid_flag_sets():
     assert len(set(ADJOINT_CANDIDATES.values())) == len(ADJOINT_CANDIDATES)
 
 
+def test_forward_candidates_are_valid_flag_sets():
+    from nefct_hip import _lib
+    from nefct_hip.autotune import FORWARD_CANDIDATES
+    allowed = _lib.F_FWD_TILE_MASK | _lib.F_FWD_VG_MASK
+    for name, fl in FORWARD_CANDIDATES.items():
+        assert fl & ~allowed == 0, name
+        assert 1 <= (fl & _lib.F_FWD_VG_MASK) >> _lib.F_FWD_VG_SHIFT <= 6, name
+    assert len(set(FORWARD_CANDIDATES.values())) == len(FORWARD_CANDIDATES) == 16
+    with pytest.raises(ValueError):
+        _lib.fwd_vg(7)
+
+
+@pytest.mark.gpu
+def test_tuned_forward_matches_default(tmp_path, monkeypatch):
+    """Every forward tile and view group gives the default's result
+    bit for bit (each ray is walked by one lane with the same arithmetic),
+    and the tuned choice is cached."""
+    torch = pytest.importorskip("torch")
+    import nefct_hip as nh
+    from nefct_hip import autotune as at
+    from nefct_hip.phantom import ellipsoid_phantom
+    monkeypatch.setenv("NEFCT_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(at, "_cache", None)
+    vol = nh.VolumeGeometry.cube((40, 36, 28), 1.5)
+    det = nh.ConeGeometry.flat(200.0, 300.0, 70, 1.4, 45, 1.4)
+    ang = torch.tensor((2 * math.pi * np.arange(40) / 40).astype(np.float32))
+    x = ellipsoid_phantom(vol, 5, 6).cuda()
+    ref = nh.project(x, vol, det, ang)
+    for name, fl in at.FORWARD_CANDIDATES.items():
+        assert torch.equal(nh.project(x, vol, det, ang, flags=fl), ref), name
+        got = nh.project(x, vol, det, ang, flags=nh.REF_FORWARD | fl)
+        assert torch.equal(got, nh.project(x, vol, det, ang, flags=nh.REF_FORWARD)), name
+    flags, rec = at.tune_forward(vol, det, ang, sample_views=32, reps=1)
+    assert rec["variant"] in at.FORWARD_CANDIDATES
+    assert torch.equal(at.project_tuned(x, vol, det, ang), ref)
+    flags2, rec2 = at.tune_forward(vol, det, ang)
+    assert flags2 == flags and rec2 == rec
+    assert at.tuned_flags(vol, det, 40, 0, "forward") == flags
+
+
 @pytest.mark.gpu
 def test_tuned_adjoint_matches_default(tmp_path, monkeypatch):
     torch = pytest.importorskip("torch")
