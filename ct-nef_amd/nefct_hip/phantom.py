@@ -2,8 +2,8 @@
 
 The ellipsoid rasteriser follows the 3-D branch of the reference's Shepp-Logan
 generator (nefct/data/sl_phantom.py:109-148): indicator of a rotated
-ellipsoid (ZXZ Euler angles, :129-145) sampled at voxel centres, intensities
-added.  Parameters are drawn from ``numpy.random.Generator(PCG64(seed))``:
+ellipsoid (ZXZ Euler angles and the alpha^T frame map of :129-147) sampled at
+voxel centres in mm, intensities added.  Parameters are drawn from ``numpy.random.Generator(PCG64(seed))``:
 semi-axes U[5, 50] mm, attenuation U[0.01, 0.03] /mm, centres U(+-0.35 size)
 per axis, Euler angles U[0, 2 pi).
 """
@@ -30,24 +30,32 @@ def ellipsoid_params(seed: int, count: int, size_mm):
 
 
 def _zxz(phi, theta, psi):
+    """The ZXZ Euler matrix of sl_phantom.py:137-143 (called alpha there)."""
     cphi, sphi = math.cos(phi), math.sin(phi)
     cth, sth = math.cos(theta), math.sin(theta)
     cpsi, spsi = math.cos(psi), math.sin(psi)
-    # rows map world coordinates to the ellipsoid frame (sl_phantom.py:137-143)
     return np.array([[cpsi * cphi - cth * sphi * spsi, cpsi * sphi + cth * cphi * spsi, spsi * sth],
                      [-spsi * cphi - cth * sphi * cpsi, -spsi * sphi + cth * cphi * cpsi, cpsi * sth],
                      [sth * sphi, -sth * cphi, cth]])
 
 
-def ellipsoid_phantom(vol: VolumeGeometry, seed: int, count: int, device="cpu", dtype=torch.float32):
-    """Volume [nz, ny, nx] of additive ellipsoids (attenuation in 1/mm)."""
+def rasterise_ellipsoids(vol: VolumeGeometry, ellipsoids, device="cpu", dtype=torch.float32):
+    """Volume [nz, ny, nx] of additive ellipsoids sampled at voxel centres.
+
+    ``ellipsoids``: iterable of (semi_axes_mm[3], mu, centre_mm[3], euler[3]),
+    centres relative to the volume centre.  A world point g (relative to the
+    ellipsoid's centre) has ellipsoid-frame coordinates alpha^T g, as in
+    sl_phantom.py:145-147 (x' = sum_i g_i alpha[i, 0], ...): semi-axis 0 lies
+    along column 0 of alpha, and so on.  Unlike the reference, whose centres
+    are given in the rotated frame, centres here are world positions."""
     nx, ny, nz = vol.grid
     vx, vy, vz = vol.voxel
     img = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
     x = torch.tensor(vol.center[0] - vol.size[0] / 2 + (np.arange(nx) + 0.5) * vx, dtype=torch.float64, device=device)
     y = torch.tensor(vol.center[1] - vol.size[1] / 2 + (np.arange(ny) + 0.5) * vy, dtype=torch.float64, device=device)
     z = torch.tensor(vol.center[2] - vol.size[2] / 2 + (np.arange(nz) + 0.5) * vz, dtype=torch.float64, device=device)
-    for axes, mu, centre, euler in ellipsoid_params(seed, count, vol.size):
+    for axes, mu, centre, euler in ellipsoids:
+        axes = np.asarray(axes, np.float64)
         R = _zxz(*euler)
         r = float(axes.max())
         # bounding box in index space
@@ -61,12 +69,17 @@ def ellipsoid_phantom(vol: VolumeGeometry, seed: int, count: int, device="cpu", 
         Y = y[lo[1]:hi[1]] - (centre[1] + vol.center[1])
         Z = z[lo[2]:hi[2]] - (centre[2] + vol.center[2])
         Zg, Yg, Xg = torch.meshgrid(Z, Y, X, indexing="ij")
-        u = Xg * R[0, 0] + Yg * R[0, 1] + Zg * R[0, 2]
-        v = Xg * R[1, 0] + Yg * R[1, 1] + Zg * R[1, 2]
-        w = Xg * R[2, 0] + Yg * R[2, 1] + Zg * R[2, 2]
+        u = Xg * R[0, 0] + Yg * R[1, 0] + Zg * R[2, 0]
+        v = Xg * R[0, 1] + Yg * R[1, 1] + Zg * R[2, 1]
+        w = Xg * R[0, 2] + Yg * R[1, 2] + Zg * R[2, 2]
         inside = (u / axes[0]) ** 2 + (v / axes[1]) ** 2 + (w / axes[2]) ** 2 <= 1.0
         img[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += inside.to(torch.float64) * mu
     return img.to(dtype)
+
+
+def ellipsoid_phantom(vol: VolumeGeometry, seed: int, count: int, device="cpu", dtype=torch.float32):
+    """Volume [nz, ny, nx] of ``count`` seeded additive ellipsoids (attenuation in 1/mm)."""
+    return rasterise_ellipsoids(vol, ellipsoid_params(seed, count, vol.size), device=device, dtype=dtype)
 
 
 def gaussian_dvf(shape_zyx, seed: int, voxel_mm=1.0, bumps=8, device="cpu"):

@@ -30,7 +30,9 @@ reference classes and its data layout (numpy-order [x, y, z] images,
 (``dist_3d_mod.so``, no source in the repository), so their arithmetic cannot
 be reproduced; these mirrors call the exact Siddon pair of this package in mm
 units (the reference's unit scaling by ``unit_size``, project.py:74-97, is
-then the identity).
+then the identity), centre the volume on the axis as the op does (no centre
+is passed to it), and pass the per-view ``offsets`` (mm) on as the helical
+lift of source and detector.
 """
 from __future__ import annotations
 
@@ -43,6 +45,7 @@ import torch
 from . import _lib
 from . import ops
 from .deform import deform_invert_tex, deform_op, deform_tex_op, rotate_tex_op, shift_tex_op
+from .formats import IOMixin
 from .geometry import ConeGeometry, VolumeGeometry
 from .siddon import backproject, project
 
@@ -152,7 +155,7 @@ deform_tex = _DeformTexOps()
 # caller mirrors (nefct/functions/*.py)
 # --------------------------------------------------------------------------
 @dataclass
-class DetectorConfig:
+class DetectorConfig(IOMixin):
     """DetectorDirectAConfig / DetectorDirectBConfig (nefct/geometry/detector_config.py:8-37)."""
     number: int
     size: float
@@ -164,7 +167,7 @@ class DetectorConfig:
 
 
 @dataclass
-class ScannerConfig3D:
+class ScannerConfig3D(IOMixin):
     """nefct/geometry/scanner_config.py:44-77: SID = source-detector, SAD = source-axis."""
     mode: str
     SID: float
@@ -179,7 +182,7 @@ class ScannerConfig3D:
 
 
 @dataclass
-class Image3D:
+class Image3D(IOMixin):
     """nefct/data/image.py:46-51: data indexed [x, y, z], box centre/size in mm."""
     data: torch.Tensor
     center: tuple
@@ -200,7 +203,7 @@ class Image3D:
 
 
 @dataclass
-class ProjectionSequence3D:
+class ProjectionSequence3D(IOMixin):
     """nefct/data/projection.py:69-113: data indexed [a, b, view]."""
     data: torch.Tensor
     scanner: ScannerConfig3D
@@ -225,9 +228,21 @@ def _to_vba(data_abv):
     return data_abv.permute(2, 1, 0).contiguous()
 
 
-def _check_offsets(offsets):
-    if offsets is not None and np.any(np.asarray(offsets) != 0):
-        raise _lib.NefctError("per-view z offsets are supported by the FDK backprojection only")
+def _op_volume(shape, size):
+    """The volume the reference callers' op sees: they pass the shape and
+    lengths but no centre (project.py:73-86, back_project.py:155-167), so the
+    volume is centred on the rotation axis whatever ``Image3D.center`` says."""
+    return VolumeGeometry(tuple(int(n) for n in shape), (0.0, 0.0, 0.0), tuple(float(v) for v in size))
+
+
+def _offsets(offsets, idx=None):
+    """Per-view z offsets (mm) for the views ``idx``, or None when absent or all zero."""
+    if offsets is None:
+        return None
+    off = np.asarray(offsets, np.float32).reshape(-1)
+    if idx is not None:
+        off = off[idx]
+    return off if np.any(off != 0) else None
 
 
 @dataclass
@@ -238,9 +253,9 @@ class Project3D:
     offsets: Optional[np.ndarray] = None
 
     def __call__(self, image: Image3D) -> ProjectionSequence3D:
-        _check_offsets(self.offsets)
         det = self.config.cone()
-        p = project(_to_zyx(image.data), image.geometry(), det, np.asarray(self.angles, np.float32))
+        p = project(_to_zyx(image.data), _op_volume(image.shape, image.size), det,
+                    np.asarray(self.angles, np.float32), offsets=_offsets(self.offsets))
         return ProjectionSequence3D(_to_abv(p), self.config, np.asarray(self.angles), self.offsets)
 
 
@@ -252,10 +267,10 @@ class BackProject3D:
     unit_size: float
 
     def __call__(self, proj: ProjectionSequence3D) -> Image3D:
-        _check_offsets(proj.offsets)
         size = [n * self.unit_size for n in self.shape]
-        vol = VolumeGeometry(tuple(self.shape), (0.0, 0.0, 0.0), tuple(size))
-        v = backproject(_to_vba(proj.data), vol, self.config.cone(), np.asarray(proj.angles, np.float32))
+        vol = _op_volume(self.shape, size)
+        v = backproject(_to_vba(proj.data), vol, self.config.cone(), np.asarray(proj.angles, np.float32),
+                        offsets=_offsets(proj.offsets))
         return Image3D(_to_xyz(v), (0.0, 0.0, 0.0), tuple(size))
 
 
@@ -271,16 +286,16 @@ class Project3DT:
     deformer: Callable
 
     def __call__(self, image: Image3D) -> ProjectionSequence3D:
-        _check_offsets(self.offsets)
         det = self.config.cone()
         ang = np.asarray(self.angles, np.float32)
         ts = np.asarray(self.timestamps)
+        vol = _op_volume(image.shape, image.size)
         out = torch.zeros((det.na, det.nb, len(ang)), dtype=torch.float32, device=image.data.device)
         for t in np.unique(ts):
             idx = np.nonzero(ts == t)[0]
             warped = self.deformer(image, t)
             data = warped.data if isinstance(warped, Image3D) else warped
-            p = project(_to_zyx(data), image.geometry(), det, ang[idx])
+            p = project(_to_zyx(data), vol, det, ang[idx], offsets=_offsets(self.offsets, idx))
             out[:, :, torch.as_tensor(idx, device=out.device)] = _to_abv(p)
         return ProjectionSequence3D(out, self.config, ang, self.offsets, ts)
 
@@ -295,18 +310,16 @@ class Project3DNT:
     timestamps: np.ndarray
 
     def __call__(self, image: Image3D) -> ProjectionSequence3D:
-        _check_offsets(self.offsets)
         det = self.config.cone()
         ang = np.asarray(self.angles, np.float32)
         ts = np.asarray(self.timestamps)
-        geo = VolumeGeometry(tuple(image.data.shape[:3]), tuple(float(c) for c in image.center),
-                             tuple(float(s) for s in image.size))
+        geo = _op_volume(image.data.shape[:3], image.size)
         out = torch.zeros((det.na, det.nb, len(ang)), dtype=torch.float32, device=image.data.device)
         for i, t in enumerate(np.asarray(image.timestamps)):
             idx = np.nonzero(ts == t)[0]
             if idx.size == 0:
                 continue
-            p = project(_to_zyx(image.data[..., i]), geo, det, ang[idx])
+            p = project(_to_zyx(image.data[..., i]), geo, det, ang[idx], offsets=_offsets(self.offsets, idx))
             out[:, :, torch.as_tensor(idx, device=out.device)] = _to_abv(p)
         return ProjectionSequence3D(out, self.config, ang, self.offsets, ts)
 
@@ -323,9 +336,8 @@ class BackProject3DT:
     deformer: Callable
 
     def __call__(self, proj: ProjectionSequence3D) -> Image3D:
-        _check_offsets(proj.offsets)
         size = [n * self.unit_size for n in self.shape]
-        vol = VolumeGeometry(tuple(self.shape), (0.0, 0.0, 0.0), tuple(size))
+        vol = _op_volume(self.shape, size)
         det = self.config.cone()
         ang = np.asarray(proj.angles, np.float32)
         ts = np.asarray(proj.timestamps)
@@ -333,7 +345,7 @@ class BackProject3DT:
         for t in np.unique(ts):
             idx = np.nonzero(ts == t)[0]
             sub = _to_vba(proj.data[:, :, torch.as_tensor(idx, device=proj.data.device)])
-            bp = _to_xyz(backproject(sub, vol, det, ang[idx]))
+            bp = _to_xyz(backproject(sub, vol, det, ang[idx], offsets=_offsets(proj.offsets, idx)))
             d = self.deformer(bp, t, self.timestamps)
             total = d if total is None else total + d
         return Image3D(total, (0.0, 0.0, 0.0), tuple(size), np.asarray(self.timestamps))
@@ -348,9 +360,8 @@ class BackProject3DNT:
     timestamps: np.ndarray
 
     def __call__(self, proj: ProjectionSequence3D) -> Image3D:
-        _check_offsets(proj.offsets)
         size = [n * self.unit_size for n in self.shape]
-        vol = VolumeGeometry(tuple(self.shape), (0.0, 0.0, 0.0), tuple(size))
+        vol = _op_volume(self.shape, size)
         det = self.config.cone()
         ang = np.asarray(proj.angles, np.float32)
         ts = np.asarray(proj.timestamps)
@@ -361,5 +372,5 @@ class BackProject3DNT:
                 phases.append(torch.zeros(self.shape, dtype=torch.float32, device=proj.data.device))
                 continue
             sub = _to_vba(proj.data[:, :, torch.as_tensor(idx, device=proj.data.device)])
-            phases.append(_to_xyz(backproject(sub, vol, det, ang[idx])))
+            phases.append(_to_xyz(backproject(sub, vol, det, ang[idx], offsets=_offsets(proj.offsets, idx))))
         return Image3D(torch.stack(phases, dim=-1), (0.0, 0.0, 0.0), tuple(size), np.asarray(self.timestamps))
