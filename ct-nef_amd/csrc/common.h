@@ -52,6 +52,15 @@ __device__ __forceinline__ double rcp64(double x)
     return fma(r, e, r);
 }
 
+// v unchanged, but opaque to the optimiser: it cannot fuse the product that
+// made v into a later addition (an FMA contraction rounds once instead of
+// twice), which it may do in one kernel and not in another.
+__device__ __forceinline__ float no_contract(float v)
+{
+    asm("" : "+v"(v));
+    return v;
+}
+
 #define NEFCT_CHECK_LAUNCH()                                        \
     do {                                                            \
         hipError_t e_ = hipGetLastError();                          \

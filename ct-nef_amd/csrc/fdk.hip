@@ -179,10 +179,12 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
 // voxels); all views accumulate in registers; one write per voxel.  Per
 // (x, y, view) the detector column, magnification and distance weight are
 // computed once in FP64 (reciprocals instead of divisions) and shared by the
-// ZC voxels of the column; the row coordinate is FP32 per voxel.  With
+// ZC voxels of the column; the row coordinate is FP32 per voxel, from the
+// chunk's first voxel and the per-voxel step: fb_k = fma(k, vz mb, fb_first),
+// one FMA per voxel and no per-voxel z in registers.  With
 // PAIR (na >= 2, so ia0 <= na - 2) the two taps of a detector row are read as
 // one 8-byte load (4-byte aligned; gfx950 loads need only dword alignment).
-constexpr int ZC = 8;
+constexpr int ZC = 16;
 
 template <bool PAIR>
 __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, const double* __restrict__ trig,
@@ -196,12 +198,10 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
     const int izc = blockIdx.z * ZC;
     if (ix >= nx || iy >= ny) return;
     const double x = x0 + (ix + 0.5) * vx, y = y0 + (iy + 0.5) * vy;
-    float acc[ZC], zf[ZC];
+    float acc[ZC];
 #pragma unroll
-    for (int k = 0; k < ZC; ++k) {
-        acc[k] = 0.0f;
-        zf[k] = (float)(z0 + (izc + k + 0.5) * vz);
-    }
+    for (int k = 0; k < ZC; ++k) acc[k] = 0.0f;
+    const float zc = (float)(z0 + (izc + 0.5) * vz), vzf = (float)vz;   // first voxel's z, step
     const int kn = min(ZC, nz - izc);
     const double ha = na * da * 0.5, hb = nb * db * 0.5;
     const double inv_da = 1.0 / da;
@@ -233,9 +233,10 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
         float tb[ZC];
         bool valid[ZC];
         const float fbv = fmaf(-off, mb, fb0);   // row coordinate of z = 0
+        const float fbs = fmaf(zc, mb, fbv), dzb = no_contract(vzf * mb);
 #pragma unroll
         for (int k = 0; k < ZC; ++k) {
-            const float fb = fmaf(zf[k], mb, fbv);
+            const float fb = fmaf((float)k, dzb, fbs);
             valid[k] = k < kn && fb >= 0.0f && fb <= nb1;
             const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
             const int ib0 = min((int)fbc, max(nb - 2, 0));   // fbc >= 0: truncation is floor
@@ -267,6 +268,248 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
     for (int k = 0; k < ZC; ++k) {
         const int iz = izc + k;
         if (iz < nz) image[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * iz)] = acc[k];
+    }
+}
+
+// Staged variant of k_fdk_bp.  A workgroup owns SB x SB voxel columns times SZ
+// voxels in z (a thread: one column's SZ voxels).  Per view it copies the
+// block's detector footprint into LDS once (the bounding rectangle of the
+// projected voxel-centre box with one cell of margin, ~1500 cells at
+// 512^3-class geometries) and reads every voxel's four taps from there,
+// instead of gathering 16 bytes per voxel and view through the vector cache
+// (~16 K taps per block and view).  The footprint of view iv + 1 is loaded
+// into registers before view iv is computed and written to the other LDS
+// buffer after it: one barrier per view.  Per voxel and view the arithmetic
+// is k_fdk_bp's, operation for operation, so both give identical images.  A
+// view whose footprint exceeds SRW x SRH cells is gathered from global
+// memory, as k_fdk_bp does.  Footprints are bounded only when every voxel
+// centre lies in front of the source (U > 0 for every view), which the
+// launcher checks.
+//
+// LDS rows have a fixed stride of SRS floats: a voxel's four taps are two
+// ds_read2_b32 at constant offsets (0, 1) and (SRS, SRS + 1) from one
+// address, and rows 4 floats apart in bank order spread lanes that read the
+// same column of different rows over different banks.  Staging: lane
+// (rsub, m) = (lane / 16, lane % 16) of wave w copies cells 4m .. 4m + 3 of
+// row 16 j + 4 w + rsub (j < SRH / 16) with one 16-byte load, whose address
+// is a per-view scalar plus a per-thread constant.
+constexpr int SB = 16;          // voxel columns per block side
+constexpr int SZ = 16;          // voxels per thread along z
+constexpr int SRW = 64;         // staged footprint cells per row, at most
+constexpr int SRH = 64;         // staged footprint rows, at most
+constexpr int SRS = 68;         // LDS row stride (floats; a multiple of 4)
+constexpr int SNJ = SRH / 16;   // staging slots per thread and view
+
+struct Foot {
+    int ua0, vb0, wq, H;        // first column and row, row length / 4, rows; wq == 0: not staged
+};
+
+// Footprints of views v0 .. v0 + 7 of the block whose voxel-centre box is
+// [cx0, cx1] x [cy0, cy1] x [cz0, cz1]: lane group g = lane / 8 takes view
+// v0 + g (views past nv give an unstaged footprint), lane & 7 a corner.  The
+// projective map sends the box into the hull of its corners' images, so the
+// corners' extreme indices bound every voxel's taps; the one-cell margin
+// covers FP32 rounding of the voxels' row coordinates (~1e-3 cells).
+__device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int v0,
+                               double cx0, double cx1, double cy0, double cy1, double cz0, double cz1, double SID,
+                               double SAD, double ha, double hb, double inv_da, double ai, double inv_db, double bi,
+                               int na, int nb)
+{
+    const int lane = threadIdx.x & 63, q = lane & 7;
+    const int iv = min(v0 + (lane >> 3), nv - 1);
+    const double c = trig[2 * iv], s = trig[2 * iv + 1];
+    const double off = offsets ? (double)offsets[iv] : 0.0;
+    const double x = (q & 1) ? cx1 : cx0, y = (q & 2) ? cy1 : cy0, z = (q & 4) ? cz1 : cz0;
+    const double U = SID + x * c + y * s;
+    const double mag = SAD * rcp64(U);
+    // the launcher guarantees U >= 1e-3 SID, so both indices are finite
+    float amn = (float)(((-x * s + y * c) * mag + ha) * inv_da - ai);
+    float bmn = (float)(((z - off) * mag + hb) * inv_db - bi);
+    float amx = amn, bmx = bmn;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        amn = fminf(amn, __shfl_xor(amn, m));
+        amx = fmaxf(amx, __shfl_xor(amx, m));
+        bmn = fminf(bmn, __shfl_xor(bmn, m));
+        bmx = fmaxf(bmx, __shfl_xor(bmx, m));
+    }
+    // index bounds clamped in float before the conversion
+    int ua0 = (int)fminf(fmaxf(floorf(amn) - 1.0f, 0.0f), (float)(na - 2));
+    int ua1 = (int)fminf(fmaxf(floorf(amx) + 2.0f, 0.0f), (float)(na - 1));
+    ua1 = max(ua1, ua0 + 1);
+    const int vb0 = (int)fminf(fmaxf(floorf(bmn) - 1.0f, 0.0f), (float)(nb - 2));
+    int vb1 = (int)fminf(fmaxf(floorf(bmx) + 2.0f, 0.0f), (float)(nb - 1));
+    vb1 = max(vb1, vb0 + 1);
+    // whole 16-byte groups: the row widened to a multiple of 4 cells, moved
+    // left where it would pass the detector's last column
+    const int ws = (ua1 - ua0 + 4) & ~3;
+    ua0 = min(ua0, na - ws);
+    const int H = vb1 - vb0 + 1;
+    const bool staged = v0 + (lane >> 3) < nv && ua0 >= 0 && ws <= SRW && H <= SRH;
+    return make_int4(ua0, vb0, staged ? ws >> 2 : 0, H);
+}
+
+// 16 bytes from a 4-byte aligned address (gfx950 global loads need only dword
+// alignment), returned by value so the destination stays in registers
+__device__ __forceinline__ float4 load16(const char* p)
+{
+    float4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+__device__ __forceinline__ Foot foot_of(const int4& grp, int g)
+{
+    Foot f;
+    f.ua0 = __builtin_amdgcn_readlane(grp.x, 8 * g);
+    f.vb0 = __builtin_amdgcn_readlane(grp.y, 8 * g);
+    f.wq = __builtin_amdgcn_readlane(grp.z, 8 * g);
+    f.H = __builtin_amdgcn_readlane(grp.w, 8 * g);
+    return f;
+}
+
+__global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restrict__ q, const double* __restrict__ trig,
+                                                       const float* __restrict__ offsets, int nv, int nx, int ny,
+                                                       int nz, double x0, double y0, double z0, double vx,
+                                                       double vy, double vz, double SID, double SAD, int na,
+                                                       double da, double ai, int nb, double db, double bi,
+                                                       float* __restrict__ image)
+{
+    __shared__ float4 lds4[2][SRH * SRS / 4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ix = blockIdx.x * SB + (lane & 15);
+    const int iy = blockIdx.y * SB + wave * 4 + (lane >> 4);
+    const int izc = blockIdx.z * SZ;
+    const bool active = ix < nx && iy < ny;
+    const int kn = min(SZ, nz - izc);
+    const double x = x0 + (ix + 0.5) * vx, y = y0 + (iy + 0.5) * vy;
+    float acc[SZ];
+#pragma unroll
+    for (int k = 0; k < SZ; ++k) acc[k] = 0.0f;
+    const float zc = (float)(z0 + (izc + 0.5) * vz), vzf = (float)vz;   // first voxel's z, step
+    const double ha = na * da * 0.5, hb = nb * db * 0.5;
+    const double inv_da = 1.0 / da, inv_db64 = 1.0 / db;
+    const float inv_db = (float)inv_db64;
+    const float fb0 = (float)(hb / db - bi);
+    const float nb1 = (float)(nb - 1);
+    const unsigned rowstep = 4u * (unsigned)na;
+    // the block's voxel-centre box
+    const int bx1 = min((int)blockIdx.x * SB + SB - 1, nx - 1), by1 = min((int)blockIdx.y * SB + SB - 1, ny - 1);
+    const double cx0 = x0 + (blockIdx.x * SB + 0.5) * vx, cx1 = x0 + (bx1 + 0.5) * vx;
+    const double cy0 = y0 + (blockIdx.y * SB + 0.5) * vy, cy1 = y0 + (by1 + 0.5) * vy;
+    // (the box spans all SZ z slots, also those past the volume, whose samples
+    // are dropped but whose clamped taps are read)
+    const double cz0 = z0 + (izc + 0.5) * vz, cz1 = z0 + (izc + SZ - 1 + 0.5) * vz;
+    auto footprints = [&](int v0) {
+        return fdk_footprints(trig, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb, inv_da, ai,
+                              inv_db64, bi, na, nb);
+    };
+    // this thread's staging cells: row 16 j + 4 wave + rsub, columns 4m .. 4m + 3
+    const int rsub = lane >> 4, m4 = 4 * (lane & 15);
+    const unsigned goff = 4u * (unsigned)(rsub * na + m4);    // bytes, within the view (< 2^31)
+    const int soff = rsub * SRS + m4;                         // floats, within an LDS buffer
+    auto stage_load = [&](const Foot& f, int jv, float4 (&st)[SNJ]) {
+        if (f.wq == 0) return;
+        const bool col_ok = m4 < 4 * f.wq;
+        const char* qb = reinterpret_cast<const char*>(q + (size_t)nb * na * jv);
+#pragma unroll
+        for (int j = 0; j < SNJ; ++j) {
+            const int r0 = 16 * j + 4 * wave;                 // wave-uniform
+            if (col_ok && r0 + rsub < f.H) st[j] = load16(qb + 4u * (unsigned)((f.vb0 + r0) * na + f.ua0) + goff);
+        }
+    };
+    auto stage_store = [&](const Foot& f, int b, const float4 (&st)[SNJ]) {
+        if (f.wq == 0) return;
+        const bool col_ok = m4 < 4 * f.wq;
+        float* lb = reinterpret_cast<float*>(&lds4[b][0]);
+#pragma unroll
+        for (int j = 0; j < SNJ; ++j) {
+            const int r0 = 16 * j + 4 * wave;
+            if (col_ok && r0 + rsub < f.H) *reinterpret_cast<float4*>(lb + r0 * SRS + soff) = st[j];
+        }
+    };
+    int4 grp = footprints(0);
+    Foot cur = foot_of(grp, 0);
+    {
+        float4 st[SNJ];
+        stage_load(cur, 0, st);
+        stage_store(cur, 0, st);
+    }
+    __syncthreads();
+    for (int iv = 0; iv < nv; ++iv) {
+        const int jn = iv + 1;
+        Foot nxt = {0, 0, 0, 0};
+        if (jn < nv) {
+            if ((jn & 7) == 0) grp = footprints(jn);
+            nxt = foot_of(grp, jn & 7);
+        }
+        float4 st[SNJ];
+        stage_load(nxt, jn, st);
+        // view iv, from LDS buffer iv & 1 (or from global memory)
+        const float* qv = q + (size_t)nb * na * iv;
+        const float off = offsets ? offsets[iv] : 0.0f;
+        const double c = trig[2 * iv], s = trig[2 * iv + 1];
+        const double U = SID + x * c + y * s;
+        if (active && U > 0.0) {
+            const double iU = rcp64(U);
+            const double mag = SAD * iU;
+            const double fa = ((-x * s + y * c) * mag + ha) * inv_da - ai;
+            if (fa >= 0.0 && fa <= (double)(na - 1)) {
+                const int ia0 = min((int)floor(fa), na - 2);
+                const float ta = (float)(fa - ia0);
+                const float wr = (float)(SID * iU);
+                const float w = wr * wr;
+                const float mb = (float)mag * inv_db;
+                const float fbv = fmaf(-off, mb, fb0);   // row coordinate of z = 0
+                const float fbs = fmaf(zc, mb, fbv), dzb = no_contract(vzf * mb);
+                if (cur.wq > 0) {
+                    // float index of row ib0's tap ia0: ib0 SRS + base
+                    const float* lb = reinterpret_cast<const float*>(&lds4[iv & 1][0]);
+                    const int base = (ia0 - cur.ua0) - cur.vb0 * SRS;
+#pragma unroll
+                    for (int k = 0; k < SZ; ++k) {
+                        const float fb = fmaf((float)k, dzb, fbs);
+                        const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
+                        const bool valid = k < kn && fb == fbc;   // 0 <= fb <= nb - 1
+                        const int ib0 = min((int)fbc, nb - 2);    // fbc >= 0: truncation is floor
+                        const float tb = fbc - (float)ib0;
+                        const float* p = lb + (__mul24(ib0, SRS) + base);
+                        const float p00 = p[0], p01 = p[1], p10 = p[SRS], p11 = p[SRS + 1];
+                        const float top = fmaf(ta, p01 - p00, p00);
+                        const float bot = fmaf(ta, p11 - p10, p10);
+                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
+                        acc[k] = valid ? sum : acc[k];
+                    }
+                } else {
+                    const char* qb = reinterpret_cast<const char*>(qv);
+#pragma unroll
+                    for (int k = 0; k < SZ; ++k) {
+                        const float fb = fmaf((float)k, dzb, fbs);
+                        const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
+                        const bool valid = k < kn && fb == fbc;
+                        const int ib0 = min((int)fbc, nb - 2);
+                        const float tb = fbc - (float)ib0;
+                        const unsigned o0 = 4u * (unsigned)(__mul24(ib0, na) + ia0), o1 = o0 + rowstep;
+                        float2 p0, p1;
+                        __builtin_memcpy(&p0, qb + o0, 8);
+                        __builtin_memcpy(&p1, qb + o1, 8);
+                        const float top = fmaf(ta, p0.y - p0.x, p0.x);
+                        const float bot = fmaf(ta, p1.y - p1.x, p1.x);
+                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
+                        acc[k] = valid ? sum : acc[k];
+                    }
+                }
+            }
+        }
+        stage_store(nxt, jn & 1, st);
+        __syncthreads();
+        cur = nxt;
+    }
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < SZ; ++k) {
+        if (k < kn) image[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * (size_t)(izc + k))] = acc[k];
     }
 }
 
@@ -328,10 +571,10 @@ int nefct_fdk_filter(const float* in, float* out, int rows, int na, const float*
 
 size_t nefct_fdk_workspace_size(int nv) { return sizeof(double) * 2 * (size_t)(nv > 0 ? nv : 1); }
 
-int nefct_fdk_backproject(const float* filtered, const int grid[3], const float center[3], const float size[3],
-                          const float* angles, const float* offsets, int nv, float SID, float SAD, int na, float da,
-                          float ai, int nb, float db, float bi, float* image, void* workspace,
-                          size_t workspace_bytes, void* stream)
+int nefct_fdk_backproject_mapped(const float* filtered, const int grid[3], const float center[3],
+                                 const float size[3], const float* angles, const float* offsets, int nv, float SID,
+                                 float SAD, int na, float da, float ai, int nb, float db, float bi, float* image,
+                                 int mapping, void* workspace, size_t workspace_bytes, void* stream)
 {
     if (!filtered || !grid || !center || !size || !angles || !image) return NEFCT_ERR_NULL;
     if (grid[0] <= 0 || grid[1] <= 0 || grid[2] <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
@@ -343,14 +586,35 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
         !finite_pos(size[1]) || !finite_pos(size[2]) || !std::isfinite(center[0]) || !std::isfinite(center[1]) ||
         !std::isfinite(center[2]) || !std::isfinite(ai) || !std::isfinite(bi))
         return NEFCT_ERR_GEOMETRY;
+    if (mapping != NEFCT_FDK_BP_AUTO && mapping != NEFCT_FDK_BP_GATHER && mapping != NEFCT_FDK_BP_STAGED)
+        return NEFCT_ERR_UNSUPPORTED;
     if (!workspace || workspace_bytes < nefct_fdk_workspace_size(nv)) return NEFCT_ERR_WORKSPACE;
-    hipStream_t s = (hipStream_t)stream;
-    double* trig = (double*)workspace;
-    hipLaunchKernelGGL(k_trig, dim3((nv + 255) / 256), dim3(256), 0, s, angles, nv, trig);
     const int nx = grid[0], ny = grid[1], nz = grid[2];
     const double vx = (double)size[0] / nx, vy = (double)size[1] / ny, vz = (double)size[2] / nz;
     const double x0 = (double)center[0] - 0.5 * size[0], y0 = (double)center[1] - 0.5 * size[1],
                  z0 = (double)center[2] - 0.5 * size[2];
+    // The staged kernel needs bounded footprints: every voxel centre at least
+    // 1e-3 SID in front of the source at every angle (U >= SID - r >= 1e-3 SID
+    // with r the largest in-plane distance of a voxel centre from the axis).
+    double r2 = 0.0;
+    for (int cxy = 0; cxy < 4; ++cxy) {
+        const double px = x0 + ((cxy & 1) ? nx - 0.5 : 0.5) * vx, py = y0 + ((cxy & 2) ? ny - 0.5 : 0.5) * vy;
+        r2 = std::max(r2, px * px + py * py);
+    }
+    const bool stageable = na >= 2 && nb >= 2 && std::sqrt(r2) <= 0.999 * (double)SID &&
+                           (ny + SB - 1) / SB <= 65535 && (nz + SZ - 1) / SZ <= 65535;
+    if (mapping == NEFCT_FDK_BP_STAGED && !stageable) return NEFCT_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    double* trig = (double*)workspace;
+    hipLaunchKernelGGL(k_trig, dim3((nv + 255) / 256), dim3(256), 0, s, angles, nv, trig);
+    if (stageable && mapping != NEFCT_FDK_BP_GATHER) {
+        const dim3 g((nx + SB - 1) / SB, (ny + SB - 1) / SB, (nz + SZ - 1) / SZ);
+        hipLaunchKernelGGL(k_fdk_bp_staged, g, dim3(256), 0, s, filtered, (const double*)trig, offsets, nv, nx, ny,
+                           nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da, (double)ai, nb,
+                           (double)db, (double)bi, image);
+        NEFCT_CHECK_LAUNCH();
+        return NEFCT_OK;
+    }
     const dim3 g((nx + 63) / 64, (ny + 3) / 4, (nz + ZC - 1) / ZC);
     if (na >= 2)
         hipLaunchKernelGGL(k_fdk_bp<true>, g, dim3(256), 0, s, filtered, (const double*)trig, offsets, nv, nx, ny, nz,
@@ -362,6 +626,15 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
                            (double)db, (double)bi, image);
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
+}
+
+int nefct_fdk_backproject(const float* filtered, const int grid[3], const float center[3], const float size[3],
+                          const float* angles, const float* offsets, int nv, float SID, float SAD, int na, float da,
+                          float ai, int nb, float db, float bi, float* image, void* workspace,
+                          size_t workspace_bytes, void* stream)
+{
+    return nefct_fdk_backproject_mapped(filtered, grid, center, size, angles, offsets, nv, SID, SAD, na, da, ai, nb,
+                                        db, bi, image, NEFCT_FDK_BP_AUTO, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

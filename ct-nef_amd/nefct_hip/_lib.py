@@ -80,6 +80,8 @@ SIGNATURES = {
     "nefct_fdk_filter": [_vp, _vp, _i, _i, _vp, _vp],
     "nefct_fdk_workspace_size": [_i],
     "nefct_fdk_backproject": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp, _sz, _vp],
+    "nefct_fdk_backproject_mapped": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _i, _vp,
+                                     _sz, _vp],
     "nefct_pixel_backproject_flat": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp],
     "nefct_pixel_backproject_cyli": [_vp, _vp, _vp, _vp, _vp, _i, _f, _f, _i, _f, _f, _i, _f, _f, _vp, _vp],
     "nefct_fdk_footprint_backproject_flat": [_vp, _vp, _vp, _vp, _i, _f, _f, _f, _f, _i, _f, _f, _i, _vp, _vp],

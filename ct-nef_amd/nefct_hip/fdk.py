@@ -107,8 +107,15 @@ def ramp_filter(q: torch.Tensor, window: str = "ram-lak") -> torch.Tensor:
     return out
 
 
+MAPPINGS = {"auto": 0, "gather": 1, "staged": 2}
+
+
 def backproject(filtered: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles,
-                offsets=None) -> torch.Tensor:
+                offsets=None, mapping: str = "auto") -> torch.Tensor:
+    """Voxel-driven weighted backprojection.  ``mapping`` picks the work
+    mapping ("auto", "gather" or "staged"; identical images)."""
+    if mapping not in MAPPINGS:
+        raise _lib.NefctError(f"unknown mapping {mapping!r}; one of {sorted(MAPPINGS)}")
     _need_device(filtered)
     filtered = filtered.contiguous()
     dev = filtered.device
@@ -128,10 +135,10 @@ def backproject(filtered: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, 
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     nx, ny, nz = vol.grid
     img = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
-    st = L.nefct_fdk_backproject(_ptr(filtered), _lib.int3(vol.grid), _lib.float3(vol.center), _lib.float3(vol.size),
-                                 _ptr(ang), _ptr(off) if off is not None else None, nv, det.SID, det.SAD, det.na,
-                                 det.da, det.ai, det.nb, det.db, det.bi, _ptr(img), _ptr(ws), wsb,
-                                 _vp(_stream_ptr(dev)))
+    st = L.nefct_fdk_backproject_mapped(_ptr(filtered), _lib.int3(vol.grid), _lib.float3(vol.center),
+                                        _lib.float3(vol.size), _ptr(ang), _ptr(off) if off is not None else None,
+                                        nv, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi,
+                                        _ptr(img), MAPPINGS[mapping], _ptr(ws), wsb, _vp(_stream_ptr(dev)))
     _lib.check(st, "fdk_backproject")
     return img
 
