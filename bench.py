@@ -59,6 +59,49 @@ def cpu_baseline(wl, img_cpu, stride):
     }
 
 
+def c1_full_leg(dev, reps=3):
+    """C1 (128^3 / 256x256 / 180 views, 10 ellipsoids) in full: the C
+    restatement of the reference algorithm (reference-mode forward + adjoint)
+    on the host cores over every view, beside the GPU pair in reference mode
+    and in exact mode on the same phantom."""
+    import numpy as np
+    import torch
+    import nefct_hip as nh
+    from nefct_hip import _lib
+    from nefct_hip.configs import c1
+    from nefct_hip.phantom import ellipsoid_phantom
+    from oracle import oracle as O
+    wl = c1()
+    d = wl.det
+    img = ellipsoid_phantom(wl.vol, wl.seed, wl.ellipsoids, device=dev)
+    ang_np = wl.angles()
+    ang = torch.tensor(ang_np, device=dev)
+    ov = O.vol_struct(wl.vol.grid, wl.vol.center, wl.vol.size)
+    od = O.det_struct(d.cyl, d.SID, d.SAD, d.na, d.da, d.ai, d.nb, d.db, d.bi)
+    img_cpu = img.cpu().numpy()
+    t0 = time.perf_counter()
+    p = O.ref_fwd(ov, od, ang_np, img_cpu, flags=O.REF_FWD_FLAGS)
+    t1 = time.perf_counter()
+    O.ref_bp(ov, od, ang_np, p, flags=O.REF_BP_FLAGS)
+    t2 = time.perf_counter()
+
+    def pair(ff, fb):
+        return nh.backproject(nh.project(img, wl.vol, d, ang, flags=ff), wl.vol, d, ang, flags=fb)
+
+    t_ref, _ = _timed(lambda: pair(_lib.REF_FORWARD, _lib.REF_ADJOINT), reps)
+    t_exact, _ = _timed(lambda: pair(0, 0), reps)
+    rays = wl.rays
+    log(f"[bench] C1 in full: CPU {t2 - t0:.2f} s, GPU reference-mode pair {t_ref * 1e3:.2f} ms")
+    return {
+        "workload": wl.name, "rays": rays, "cores": O.threads(),
+        "cpu_grays_s": round(rays / (t2 - t0) / 1e9, 6),
+        "cpu_s": {"forward": round(t1 - t0, 3), "adjoint": round(t2 - t1, 3)},
+        "gpu_reference_mode_pair_grays_s": round(rays / t_ref / 1e9, 4),
+        "gpu_exact_pair_grays_s": round(rays / t_exact / 1e9, 4),
+        "gpu_over_cpu_reference_mode": round((t2 - t0) / t_ref, 1),
+    }
+
+
 def batched_leg(dev, iters, log_every=True):
     """64 uneven differentiable problems (sides 64-256, 60-360 views, detector
     2 side, SOD 500-1000 mm, own phantom and Gaussian-bump warp field each):
@@ -292,6 +335,7 @@ def fdk_leg(wl, img, sino, ang, reps=2):
     t_w, q = _timed(lambda: F.preweight(p, det, parker=True, dang_rad=dang, direction=1, scale=scale), reps)
     t_f, f = _timed(lambda: F.ramp_filter(q, "ram-lak"), reps)
     t_b, rec = _timed(lambda: F.backproject(f, vol, det, a), reps)
+    t_bg, _ = _timed(lambda: F.backproject(f, vol, det, a, mapping="gather"), reps)
     nvox = vol.grid[0] * vol.grid[1] * vol.grid[2]
     # inner cylinder (radius 0.4 of the width), central half of the slices: no truncation or
     # cone-beam edge effects
@@ -310,6 +354,8 @@ def fdk_leg(wl, img, sino, ang, reps=2):
         "stage_ms": {"preweight": round(t_w * 1e3, 3), "filter": round(t_f * 1e3, 3),
                      "backproject": round(t_b * 1e3, 3)},
         "backproject_voxel_views_per_s": nvox * nv / t_b,
+        "backproject_mapping": "staged footprints (auto)",
+        "backproject_gather_ms": round(t_bg * 1e3, 3),
         "backproject_hbm": {"algorithmic_bytes": bp_bytes, "achieved_GBs": round(bp_bytes / t_b / 1e9, 1),
                             "frac": round(bp_bytes / t_b / 1e9 / HBM_PEAK_GBS, 4)},
         "filter_gflops": round(2.0 * p.numel() * det.na / t_f / 1e9, 1),
@@ -353,8 +399,8 @@ def main():
                     help="profiling only: project the first N views instead of all 720 (not the headline number)")
     ap.add_argument("--adj-flags", type=int, default=0,
                     help="profiling only: adjoint algorithm flags (16 deterministic, 32 slab windows)")
-    ap.add_argument("--only", choices=["pair", "fwd", "adj"], default="pair",
-                    help="profiling only: run one direction")
+    ap.add_argument("--only", choices=["pair", "fwd", "adj", "fdk"], default="pair",
+                    help="profiling only: run one direction, or only the FDK leg")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default launch shapes instead of the autotuner's (tuning is untimed)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain and C4 legs")
@@ -395,6 +441,10 @@ def main():
     if args.views:
         ang = ang[: args.views].contiguous()
     rays = ang.numel() * det.na * det.nb
+    if args.only == "fdk":
+        log("[bench] FDK leg only")
+        print(json.dumps({"fdk": fdk_leg(wl, img, nh.project(img, vol, det, ang), ang)}), flush=True)
+        return
 
     log("[bench] counting intersections")
     counts = nh.count_intersections(vol, det, ang, device=dev)
@@ -476,6 +526,8 @@ def main():
         log(f"[bench] CPU baseline on views 0::{args.cpu_stride}")
         cpu = cpu_baseline(wl, img.cpu(), args.cpu_stride)
         log(f"[bench] CPU baseline {cpu['value']:.4f} Grays/s on {cpu['cores']} threads")
+        if not args.views and args.only == "pair":
+            cpu["c1_full"] = c1_full_leg(dev)
 
     # HBM bytes per launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (scaled by a known-size copy) and committed; with this run's kernel
@@ -496,6 +548,26 @@ def main():
                                   "frac_of_copy": round(gbs / copy_bw["GBs"], 4) if copy_bw else None}
         if dominant in measured:
             traffic = measured[dominant]["traffic_bytes"]
+
+    # VALU instructions per launch, measured with rocprofv3 SQ counters and
+    # committed; with this run's kernel times they give the issue rate against
+    # one wave64 instruction per SIMD per 4 cycles (plain FP32 and integer
+    # ops; conversions, v_fract and compares partly co-issue, so above 1 is
+    # possible)
+    valu = None
+    spath = os.path.join(ROOT, "profiles", "siddon_pair_c2_sq.json")
+    if not args.views and os.path.exists(spath):
+        with open(spath) as f:
+            sq = json.load(f)
+        peak = sq["issue_peak"]["wave_instructions_per_s"]
+        valu = {"peak_wave_instr_per_s": peak}
+        for name, t_k in (("siddon_forward", t_fwd), ("siddon_adjoint", t_adj)):
+            k = sq["kernels"].get(name)
+            if k:
+                rate = k["SQ_INSTS_VALU"] / t_k
+                valu[name] = {"valu_instr": k["SQ_INSTS_VALU"], "G_wave_instr_per_s": round(rate / 1e9, 1),
+                              "issue_frac": round(rate / peak, 3),
+                              "lane_valu_per_intersection": round(64 * k["SQ_INSTS_VALU"] / n_int, 2)}
 
     rec = {
         "metric": "Siddon fwd+adjoint Grays/s at 512x512x256, 720 views",
@@ -541,6 +613,7 @@ def main():
                     "HBM traffic; both Siddon kernels are bound by instruction issue",
         },
         "hbm_measured": measured or None,
+        "valu_issue": valu,
         "copy_bandwidth": copy_bw,
         "batched": batched,
         "c4": c4,
