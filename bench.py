@@ -111,6 +111,7 @@ def batched_leg(dev, iters, log_every=True):
     path runs one persistent launch per direction over all problems."""
     import torch
     import nefct_hip as nh
+    from nefct_hip import _lib
     from nefct_hip.batch import SiddonBatch, batched_project
     from nefct_hip.configs import c5
     from nefct_hip.deform import warp
@@ -126,6 +127,11 @@ def batched_leg(dev, iters, log_every=True):
     g = torch.Generator(device=dev).manual_seed(55)
     ys = [torch.randn(s.shape, generator=g, device=dev) for s in batch.sinograms]
     rays = batch.rays
+    for dst, y in zip(batch.sinograms, ys):
+        dst.copy_(y)
+    tuned = batch.tune()             # batched adjoint lane spacing, untimed
+    shape = batch.flags & _lib.F_LANE_STRIDE2   # the plain calls get the same lane spacing
+    log(f"[bench] batched leg: tuned adjoint {tuned}")
 
     def plain(with_warp):
         for p, b, d, a, y in zip(probs, bases, dvfs, angs, ys):
@@ -135,7 +141,7 @@ def batched_leg(dev, iters, log_every=True):
                 v = warp(x, dv, 1)
             else:
                 v = x
-            s = nh.forward_project(v, p.vol, p.det, a)
+            s = nh.forward_project(v, p.vol, p.det, a, flags=shape)
             (s * y).sum().backward()
 
     def batched(with_warp):
@@ -170,6 +176,7 @@ def batched_leg(dev, iters, log_every=True):
         "plain_warp_pair": round(res["plain_warp_pair"], 5),
         "batched_warp_pair": round(res["batched_warp_pair"], 5),
         "speedup_warp_pair": round(res["batched_warp_pair"] / res["plain_warp_pair"], 3),
+        "adjoint_lane_spacing": tuned,
         "iters": iters,
     }
 
@@ -185,6 +192,7 @@ def c4_leg(dev, iters=3):
     (warps, batched pair, residual, transposed warps, update) replayed."""
     import torch
     import nefct_hip as nh
+    from nefct_hip import _lib
     from nefct_hip.batch import SiddonBatch
     from nefct_hip.configs import c4
     from nefct_hip.deform import deform_adjoint_op, deform_op, warp
@@ -201,13 +209,18 @@ def c4_leg(dev, iters=3):
     g = torch.Generator(device=dev).manual_seed(56)
     ys = [torch.randn(s.shape, generator=g, device=dev) for s in batch.sinograms]
     rays = batch.rays
+    for dst, y in zip(batch.sinograms, ys):
+        dst.copy_(y)
+    tuned = batch.tune()             # batched adjoint lane spacing, untimed
+    shape = batch.flags & _lib.F_LANE_STRIDE2   # the plain calls get the same lane spacing
+    log(f"[bench] C4 leg: tuned adjoint {tuned}")
 
     def plain():
         x = base.detach().requires_grad_(True)
         ds = [d.detach().requires_grad_(True) for d in dvfs]
         loss = 0.0
         for p, d, a, y in zip(phases, ds, angs, ys):
-            loss = loss + (nh.forward_project(warp(x, d, 1), p.vol, p.det, a) * y).sum()
+            loss = loss + (nh.forward_project(warp(x, d, 1), p.vol, p.det, a, flags=shape) * y).sum()
         loss.backward()
 
     def batched():
@@ -276,6 +289,7 @@ def c4_leg(dev, iters=3):
         "ms_batched": round(res["batched_warp_pair"][1] * 1e3, 2),
         "graph_iteration_ms": round(t_graph * 1e3, 2),
         "graph_iteration_grays_s": round(rays / t_graph / 1e9, 5),
+        "adjoint_lane_spacing": tuned,
         "iters": iters,
     }
 

@@ -1898,7 +1898,7 @@ __global__ __launch_bounds__(256) void k_batch_absmax(const nefct_problem* __res
 // persistent loop over an atomic queue holds enough extra state across
 // iterations to make this kernel spill at 16 wavefronts; the blocks are equal
 // in size within a problem, so the dispatcher balances them as well.)
-template <bool REF>
+template <bool REF, int CS>
 __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                         long long total, char* ws_base, unsigned flags)
 {
@@ -1916,7 +1916,7 @@ __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __r
     const int bz = __builtin_amdgcn_readfirstlane(local / (pr.blocks[0] * pr.blocks[1]));
     const Launch& L = batch_launches(ws_base)[p];
     const int es = fixed_exp(((const unsigned*)(ws_base + batch_max_off(nprob)))[p], L.sbound);
-    adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, 1>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
+    adj_tile_block<REF, true, BTX, BTY, BTZ, BNW, CS>(L, pr.sinogram, const_cast<float*>(pr.volume), bx, by, bz, acc,
                                                      shadows, es, false);
 }
 
@@ -2190,10 +2190,19 @@ int nefct_batch_backproject(const nefct_problem* host, const nefct_problem* dev,
     if (total_blocks <= 0) return NEFCT_OK;
     if (total_blocks > 65536ll * 65535ll) return NEFCT_ERR_SHAPE;
     const dim3 g((unsigned)std::min<long long>(total_blocks, 65536), (unsigned)((total_blocks + 65535) / 65536));
-    if (flags & NEFCT_F_REF_SLAB)
-        hipLaunchKernelGGL((k_batch_adj<true>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
-    else
-        hipLaunchKernelGGL((k_batch_adj<false>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+    // lanes one or two detector rows apart (NEFCT_F_LANE_STRIDE2), as in the plain adjoint
+    const bool cs2 = (flags & NEFCT_F_LANE_STRIDE2) != 0;
+    if (flags & NEFCT_F_REF_SLAB) {
+        if (cs2)
+            hipLaunchKernelGGL((k_batch_adj<true, 2>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+        else
+            hipLaunchKernelGGL((k_batch_adj<true, 1>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+    } else {
+        if (cs2)
+            hipLaunchKernelGGL((k_batch_adj<false, 2>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+        else
+            hipLaunchKernelGGL((k_batch_adj<false, 1>), g, dim3(64 * BNW), 0, s, dev, nprob, total_blocks, ws, flags);
+    }
     NEFCT_CHECK_LAUNCH();
     return NEFCT_OK;
 }

@@ -101,6 +101,29 @@ class SiddonBatch:
         _lib.check(st, "batch_project")
         return self.sinograms
 
+    def tune(self, reps: int = 2) -> dict:
+        """Times the batched adjoint with neighbouring lanes one and two
+        detector rows apart (identical results) on the batch's current
+        sinograms and keeps the faster for ``backproject``.  Returns the
+        record (ms per call of each)."""
+        base = self.flags & ~_lib.F_LANE_STRIDE2
+        times = {}
+        for name, fl in (("stride1", base), ("stride2", base | _lib.F_LANE_STRIDE2)):
+            self.flags = fl
+            self.backproject()
+            torch.cuda.synchronize(self.device)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                self.backproject()
+            e1.record()
+            torch.cuda.synchronize(self.device)
+            times[name] = round(e0.elapsed_time(e1) / reps, 3)
+        best = min(times, key=times.get)
+        self.flags = base | (_lib.F_LANE_STRIDE2 if best == "stride2" else 0)
+        return {"adjoint": best, "ms": times}
+
     def backproject(self):
         L = _bind()
         st = L.nefct_batch_backproject(ctypes.addressof(self.host), ctypes.c_void_p(self.table.data_ptr()), self.nprob,

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import torch
 
-from . import ops
+from . import _lib, ops
 from ._lib import REF_ADJOINT, REF_FORWARD  # noqa: F401  (re-exported)
 from .geometry import ConeGeometry, VolumeGeometry
 
@@ -58,46 +58,61 @@ def count_intersections(vol: VolumeGeometry, det: ConeGeometry, angles, device="
                             det.cyl, 0)
 
 
+# launch-shape flags: they change how the work maps onto the chip, not the
+# exact pair's result (up to summation order for the float-atomic adjoint)
+SHAPE_FLAGS = (_lib.F_TILE_MASK | _lib.F_LANE_STRIDE2 | _lib.F_FWD_TILE_MASK | _lib.F_FWD_VG_MASK
+               | _lib.F_ADJ_FLOAT_ATOMIC | _lib.F_ADJ_WINDOW | _lib.F_DETERMINISTIC)
+
+
+def _shape_flags(flags: int) -> int:
+    flags = int(flags)
+    if flags & ~SHAPE_FLAGS:
+        raise _lib.NefctError(f"autograd projections take launch-shape flags only, got {flags:#x}")
+    return flags
+
+
 class SiddonProjection(torch.autograd.Function):
     """y = A x with A the exact Siddon projector; backward applies A^T with the
-    hand-written adjoint kernel."""
+    hand-written adjoint kernel.  ``flags`` holds launch-shape flags only."""
 
     @staticmethod
-    def forward(ctx, image, vol, det, angles, offsets=None):
-        ctx.geom = (vol, det, offsets)
+    def forward(ctx, image, vol, det, angles, offsets=None, flags=0):
+        ctx.geom = (vol, det, offsets, flags)
         ctx.save_for_backward(angles)
-        return project(image, vol, det, angles, 0, offsets)
+        return project(image, vol, det, angles, flags, offsets)
 
     @staticmethod
     def backward(ctx, grad):
         (angles,) = ctx.saved_tensors
-        vol, det, offsets = ctx.geom
-        gi = backproject(grad.contiguous(), vol, det, angles, 0, offsets) if ctx.needs_input_grad[0] else None
-        return gi, None, None, None, None
+        vol, det, offsets, flags = ctx.geom
+        gi = backproject(grad.contiguous(), vol, det, angles, flags, offsets) if ctx.needs_input_grad[0] else None
+        return gi, None, None, None, None, None
 
 
 class SiddonBackprojection(torch.autograd.Function):
-    """x = A^T y; backward applies A."""
+    """x = A^T y; backward applies A.  ``flags`` holds launch-shape flags only."""
 
     @staticmethod
-    def forward(ctx, sino, vol, det, angles, offsets=None):
-        ctx.geom = (vol, det, offsets)
+    def forward(ctx, sino, vol, det, angles, offsets=None, flags=0):
+        ctx.geom = (vol, det, offsets, flags)
         ctx.save_for_backward(angles)
-        return backproject(sino, vol, det, angles, 0, offsets)
+        return backproject(sino, vol, det, angles, flags, offsets)
 
     @staticmethod
     def backward(ctx, grad):
         (angles,) = ctx.saved_tensors
-        vol, det, offsets = ctx.geom
-        gs = project(grad.contiguous(), vol, det, angles, 0, offsets) if ctx.needs_input_grad[0] else None
-        return gs, None, None, None, None
+        vol, det, offsets, flags = ctx.geom
+        gs = project(grad.contiguous(), vol, det, angles, flags, offsets) if ctx.needs_input_grad[0] else None
+        return gs, None, None, None, None, None
 
 
-def forward_project(image, vol, det, angles, offsets=None):
+def forward_project(image, vol, det, angles, offsets=None, flags: int = 0):
+    """Differentiable exact projection; ``flags`` may select launch shapes
+    (e.g. an autotuned choice), never the reference rule."""
     a = _angles(angles, image.device)
-    return SiddonProjection.apply(image, vol, det, a, _offsets(offsets, image.device))
+    return SiddonProjection.apply(image, vol, det, a, _offsets(offsets, image.device), _shape_flags(flags))
 
 
-def back_project(sino, vol, det, angles, offsets=None):
+def back_project(sino, vol, det, angles, offsets=None, flags: int = 0):
     a = _angles(angles, sino.device)
-    return SiddonBackprojection.apply(sino, vol, det, a, _offsets(offsets, sino.device))
+    return SiddonBackprojection.apply(sino, vol, det, a, _offsets(offsets, sino.device), _shape_flags(flags))

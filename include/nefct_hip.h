@@ -223,7 +223,9 @@ int nefct_batch_project(const nefct_problem* host_problems, const nefct_problem*
 
 /* Exact (or reference-mode) adjoint of every problem: one max-|projection|
  * pass, then one launch with a workgroup per 32x32x16 voxel block.  Every
- * output volume is fully written. */
+ * output volume is fully written.  NEFCT_F_LANE_STRIDE2 spaces a wavefront's
+ * lanes two detector rows apart, as in the plain adjoint (identical result);
+ * the other launch-shape flags do not apply to the batched kernels. */
 int nefct_batch_backproject(const nefct_problem* host_problems, const nefct_problem* device_problems, int nprob,
                             long long total_blocks, unsigned flags, void* workspace, size_t workspace_bytes,
                             void* stream);

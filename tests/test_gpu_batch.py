@@ -152,3 +152,47 @@ def test_warp_project_batch_gradients():
     fd = (f(dvfs[k].detach() + e) - f(dvfs[k].detach() - e)) / (2 * eps)
     an = (dvfs[k].grad[2].double()).sum() * 0 + (dvfs[k].grad[2].double() * (e[2] / eps)).sum()
     assert abs(fd.item() - an.item()) <= 2e-3 * abs(an.item()) + 1e-6
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_batched_adjoint_lane_stride_bitwise(signed):
+    """Neighbouring lanes two detector rows apart (NEFCT_F_LANE_STRIDE2) in
+    the batched adjoint: the fixed-point sums are order-independent, so the
+    volumes are bitwise those of lane stride 1; tune() keeps one of the two."""
+    from nefct_hip import _lib
+    from nefct_hip.batch import SiddonBatch
+    probs = _problems(5, 5)
+    b = SiddonBatch(probs)
+    g = torch.Generator().manual_seed(6)
+    for s in b.sinograms:
+        s.copy_(torch.randn(s.shape, generator=g) if signed else torch.rand(s.shape, generator=g))
+    b.backproject()
+    one = [v.clone() for v in b.volumes]
+    b.flags |= _lib.F_LANE_STRIDE2
+    b.backproject()
+    assert all(torch.equal(a, v) for a, v in zip(one, b.volumes))
+    rec = b.tune()
+    assert rec["adjoint"] in ("stride1", "stride2") and set(rec["ms"]) == {"stride1", "stride2"}
+    b.backproject()
+    assert all(torch.equal(a, v) for a, v in zip(one, b.volumes))
+
+
+def test_autograd_launch_shape_flags():
+    """forward_project / back_project take launch-shape flags (the pair's
+    result is unchanged: bitwise for the fixed-point adjoint) and refuse the
+    reference rule, which is not an adjoint pair."""
+    import nefct_hip as nh
+    from nefct_hip import _lib
+    vol, det, ang = _problems(7, 1)[0]
+    a = torch.tensor(ang)
+    g = torch.Generator().manual_seed(8)
+    x0 = torch.rand(vol.shape_zyx, generator=g).to(DEV)
+    y = torch.randn((len(ang), det.nb, det.na), generator=g).to(DEV)
+    grads = []
+    for fl in (0, _lib.F_LANE_STRIDE2 | _lib.F_TILE_32x16x8, _lib.F_FWD_TILE_32x8 | _lib.fwd_vg(3)):
+        x = x0.clone().requires_grad_(True)
+        (nh.forward_project(x, vol, det, a, flags=fl) * y).sum().backward()
+        grads.append(x.grad)
+    assert torch.equal(grads[0], grads[1]) and torch.equal(grads[0], grads[2])
+    with pytest.raises(_lib.NefctError):
+        nh.forward_project(x0, vol, det, a, flags=_lib.REF_FORWARD)
