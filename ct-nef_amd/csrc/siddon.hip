@@ -128,11 +128,16 @@ __device__ __forceinline__ int ceil_clamp(float v, int lo, int hi)
     return (int)fminf(fmaxf(ceilf(v), (float)lo), (float)hi);
 }
 
+// The ray set-up (ray_points, clip_axis, setup_ray_g) is evaluated by the
+// forward, the adjoints and the counter in different inlining contexts; with
+// FMA contraction off and the fused operations written out, every context
+// rounds identically, which keeps the pair an exact transpose.
 __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, int ia, int ib,
                                            double S[3], double D[3], double dz = 0.0)
 {
-    const double a = (g.ai + ia) * g.da - g.na * g.da * 0.5;
-    const double b = (g.bi + ib) * g.db - g.nb * g.db * 0.5;
+#pragma clang fp contract(off)
+    const double a = fma(g.ai + ia, g.da, -(g.na * g.da * 0.5));
+    const double b = fma(g.bi + ib, g.db, -(g.nb * g.db * 0.5));
     double lx, ly;
     if (g.cyl) {
         lx = g.sad * cos(a) - g.sid;
@@ -144,8 +149,8 @@ __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, 
     S[0] = -g.sid * c;
     S[1] = -g.sid * s;
     S[2] = dz;
-    D[0] = lx * c - ly * s;
-    D[1] = lx * s + ly * c;
+    D[0] = fma(lx, c, -(ly * s));
+    D[1] = fma(lx, s, ly * c);
     D[2] = b + dz;
 }
 
@@ -154,6 +159,7 @@ __device__ __forceinline__ void ray_points(const DetDev& g, double c, double s, 
 __device__ __forceinline__ void clip_axis(double k, double inv_k, double A, double lo, double hi, double& tin,
                                           double& tout)
 {
+#pragma clang fp contract(off)
     if (k == 0.0) {
         if (!(A > lo && A < hi)) { tin = 1.0; tout = 0.0; }
         return;
@@ -168,9 +174,10 @@ __device__ __forceinline__ void clip_axis(double k, double inv_k, double A, doub
 // What the set-up needs of walk axis m and its transverse axes a, b: the
 // volume's extents and voxel sizes and the view's source in voxel units.
 struct AxisGeom {
-    int nm, na, nb;
     double vsm, vsa, vsb, ivsm;
     double uSm, uSa, uSb;
+    int nm, na, nb;
+    int pad_[17];   // 128 bytes: a table entry's LDS offset is a shift of the axis index
 };
 
 // The view's source in voxel units (the S of ray_points).
@@ -185,31 +192,46 @@ __device__ __forceinline__ void source_voxels(const VolDev& V, const DetDev& G, 
 // 'ref' selects the reference's x/y-only dominance rule.  geom(m) supplies the
 // AxisGeom of the ray's walk axis (selected from V per lane, or read from a
 // table), with the same values either way.
-template <class Geom>
+// EARLY_EXIT returns at a miss with the fields zeroed (fewer registers: the
+// forward keeps 8 waves per SIMD); without it the set-up is straight-line
+// (fewer instructions in the adjoint's per-block set-up).  Both give the same
+// fields for a ray that hits.
+template <bool EARLY_EXIT = false, class Geom>
 __device__ __forceinline__ void setup_ray_g(const VolDev& V, const DetDev& G, double c, double s, int ia, int ib,
                                             unsigned flags, bool adjoint, Geom&& geom, Ray& r, double dz = 0.0)
 {
+#pragma clang fp contract(off)
     double S[3], D[3], d[3];
     ray_points(G, c, s, ia, ib, S, D, dz);
     for (int i = 0; i < 3; ++i) d[i] = (D[i] - S[i]) * V.inv_vs[i];
-    r.dom = -1;
-    r.s0 = r.s1 = 0;
-    r.tc = 0;
-    r.ka = r.kb = r.Ac = r.Bc = r.inv_ka = r.inv_kb = r.pa = r.pb = 0.0f;
-    r.scale = 0.0f;
+    if constexpr (EARLY_EXIT) {
+        r.dom = -1;
+        r.s0 = r.s1 = 0;
+        r.tc = 0;
+        r.ka = r.kb = r.Ac = r.Bc = r.inv_ka = r.inv_kb = r.pa = r.pb = 0.0f;
+        r.scale = 0.0f;
+    }
     int m;
+    bool hit = true;
     const bool ref = (flags & NEFCT_F_REF_SLAB) != 0;
     // reference quantities in x-voxel units (siddon.cu.cc:19-38)
     const double xd = (D[0] - S[0]) * V.inv_vs[0], yd = (D[1] - S[1]) * V.inv_vs[0], zd = (D[2] - S[2]) * V.inv_vs[0];
     if (ref) {
-        if (adjoint && (flags & NEFCT_F_SHORT_SKIP) && sqrt(xd * xd + yd * yd) < 10.0) return;
+        if (adjoint && (flags & NEFCT_F_SHORT_SKIP) && sqrt(fma(yd, yd, xd * xd)) < 10.0) hit = false;
+        if (EARLY_EXIT && !hit) return;
         const bool xdom = (adjoint && (flags & NEFCT_F_TIE_GT)) ? (fabs(xd) > fabs(yd)) : (fabs(xd) >= fabs(yd));
         m = xdom ? 0 : 1;
     } else {
         const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
         m = (ax >= ay && ax >= az) ? 0 : ((ay >= az) ? 1 : 2);
     }
-    if (d[m] == 0.0) return;
+    // Straight-line from here (without EARLY_EXIT): a ray that misses
+    // (d[m] == 0, or no slab of the box) computes on with values that are not
+    // used, and the fields are written once at the end (early returns make the
+    // compiler materialise every field's value on every exit path, a dozen
+    // moves per ray).
+    hit = hit && d[m] != 0.0;
+    if (EARLY_EXIT && !hit) return;
     const int a = (m == 0) ? 1 : 0;
     const int b = (m == 2) ? 1 : 2;
     const AxisGeom g = geom(m);
@@ -217,39 +239,45 @@ __device__ __forceinline__ void setup_ray_g(const VolDev& V, const DetDev& G, do
     const double ka = d[a] * inv_dm, kb = d[b] * inv_dm;
     const double ika = (d[a] != 0.0) ? d[m] * rcp64(d[a]) : 0.0;
     const double ikb = (d[b] != 0.0) ? d[m] * rcp64(d[b]) : 0.0;
-    const double Aa = g.uSa - ka * g.uSm, Ab = g.uSb - kb * g.uSm;
+    const double Aa = fma(-ka, g.uSm, g.uSa), Ab = fma(-kb, g.uSm, g.uSb);
     // The exact walk clips to the volume box.  The reference rule can credit an
     // in-range voxel from a slab whose exact segment lies just outside the box
     // (the Q2 attribution at a corner), so reference mode clips to the box
     // widened by one cell on each transverse side; its per-voxel range tests
-    // then decide exactly as siddon.cu.cc:80-176 does.
+    // then decide exactly as siddon.cu.cc:80-176 does.  (tin and tout stay
+    // finite: fmax and fmin drop NaN operands, and start from 0 and nm.)
     const double mg = ref ? 1.0 : 0.0;
     double tin = 0.0, tout = (double)g.nm;
     clip_axis(ka, ika, Aa, -mg, g.na + mg, tin, tout);
     clip_axis(kb, ikb, Ab, -mg, g.nb + mg, tin, tout);
-    if (!(tout > tin)) return;
+    hit = hit && tout > tin;
+    if (EARLY_EXIT && !hit) return;
     int s0 = (int)floor(tin), s1 = (int)ceil(tout);
     s0 = max(s0, 0);
     s1 = min(s1, g.nm);
-    if (s1 <= s0) return;
+    hit = hit && s1 > s0;
+    if (EARLY_EXIT && !hit) return;
     const int tc = (s0 + s1) >> 1;
     const int ha = g.na >> 1, hb = g.nb >> 1;
-    r.dom = m;
+    r.dom = hit ? m : -1;
     r.s0 = s0;
     r.s1 = s1;
     r.tc = tc;
     r.ka = (float)ka;
     r.kb = (float)kb;
-    r.Ac = (float)(Aa + ka * tc - ha);
-    r.Bc = (float)(Ab + kb * tc - hb);
+    r.Ac = (float)(fma(ka, (double)tc, Aa) - ha);
+    r.Bc = (float)(fma(kb, (double)tc, Ab) - hb);
     r.inv_ka = (float)ika;
     r.inv_kb = (float)ikb;
     r.pa = ka > 0.0 ? 1.0f : 0.0f;
     r.pb = kb > 0.0 ? 1.0f : 0.0f;
-    const double qa = ka * g.vsa * g.ivsm, qb = kb * g.vsb * g.ivsm;
-    double sc = g.vsm * (double)sqrtf((float)(1.0 + qa * qa + qb * qb));   // rounded to FP32 below
-    if (flags & NEFCT_F_INV_L2) sc = sc / (xd * xd + yd * yd + zd * zd);
-    r.scale = (float)sc;
+    const double qa = (ka * g.vsa) * g.ivsm, qb = (kb * g.vsb) * g.ivsm;
+    // the hardware square root (within 1 ulp; the argument is >= 1), the
+    // product rounded to FP32 below
+    double sc = g.vsm * (double)__builtin_amdgcn_sqrtf((float)fma(qb, qb, fma(qa, qa, 1.0)));
+    if (flags & NEFCT_F_INV_L2) sc = sc / fma(zd, zd, fma(yd, yd, xd * xd));
+    // a missing ray's scale is 0: the forward writes 0 * scale for it
+    r.scale = hit ? (float)sc : 0.0f;
 }
 
 // AxisGeom of walk axis m from the volume record.
@@ -275,7 +303,7 @@ __device__ __forceinline__ void setup_ray(const VolDev& V, const DetDev& G, doub
 {
     double uS[3];
     source_voxels(V, G, c, s, uS, dz);
-    setup_ray_g(V, G, c, s, ia, ib, flags, adjoint, [&](int m) { return axis_geom(V, uS, m); }, r, dz);
+    setup_ray_g<true>(V, G, c, s, ia, ib, flags, adjoint, [&](int m) { return axis_geom(V, uS, m); }, r, dz);
 }
 
 __device__ __forceinline__ Walk walk_of(int dom, const VolDev& V, const Layouts& Ly)
