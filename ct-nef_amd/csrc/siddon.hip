@@ -1119,6 +1119,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
     __shared__ BlockAxis bax[3];
     __shared__ AxisGeom vax[3];          // volume part of each walk axis's AxisGeom
     __shared__ double src_tab[SHV][3];   // source in voxel units per view of the chunk
+    __shared__ unsigned mh_tab[SHV];     // ceil(2^32 / H) of each view's shadow height H (cell dealing)
     if (tid < 3) {
         double zero[3] = {0.0, 0.0, 0.0};
         vax[tid] = axis_geom(P.V, zero, tid);
@@ -1156,6 +1157,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 const int4 sh = shadow_of_box(P.V, P.G, cj, sj, b0, b1, !REF, tid & 7, dzj);
                 if ((tid & 7) == 0) {
                     shadows[e] = sh;
+                    // a 64-bit division once per block and view (not in every
+                    // wavefront for every view)
+                    const unsigned Hh = (sh.w >= sh.z) ? (unsigned)(sh.w - sh.z + 1) : 1u;
+                    mh_tab[e] = (Hh > 1u) ? (unsigned)((0x100000000ull + Hh - 1) / Hh) : 0u;
                     source_voxels(P.V, P.G, cj, sj, src_tab[e], dzj);
                 }
             }
@@ -1180,7 +1185,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         const int wv = tid >> 6, ln = tid & 63;
         const unsigned W = (unsigned)(ia1 - ia0 + 1), H = (unsigned)(ib1 - ib0 + 1);
         const unsigned ncell = W * H;
-        const unsigned mH = (H > 1u) ? (unsigned)((0x100000000ull + H - 1) / H) : 0u;   // ceil(2^32 / H)
+        const unsigned mH = __builtin_amdgcn_readfirstlane(mh_tab[slot]);   // ceil(2^32 / H), 0 for H = 1
         const unsigned groups = (ncell + 64u * CS - 1) / (64u * CS) * CS;   // 64-cell groups to deal out
         const int nk = (int)((groups + NW - 1) / NW);
         auto cell_of = [&](int k, int& ia, int& ib) {
