@@ -961,14 +961,22 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_adj(Launch P, const float* _
 // several times slower than integer ones on gfx950, which is why this exists.
 // A non-finite projection value makes the affected blocks NaN.
 // ---------------------------------------------------------------------------
-template <int TX, int TY, int TZ, typename T>
+template <int TX, int TY, int TZ, typename T, bool HALO_ = false>
 struct TileAcc {
     // odd row and plane strides (in elements): lanes on consecutive y rows or
     // z planes hit distinct banks
+    static constexpr bool HALO = HALO_;
     static constexpr int SX = TX + 1;
-    static constexpr int SXY = TY * SX + ((TY * SX) % 2 == 0 ? 1 : 0);
-    T v[TZ * SXY];
-    __device__ T& at(int x, int y, int z) { return v[z * SXY + y * SX + x]; }
+    static constexpr int SXY0 = (HALO ? TY + 1 : TY) * SX;
+    static constexpr int SXY = SXY0 + (SXY0 % 2 == 0 ? 1 : 0);
+    // With HALO, every cell (x, y, z) with x in [-1, TX], y in [-1, TY], z in
+    // [-1, TZ] has a word of its own or shares one with another such halo cell
+    // (x = TX of row y is x = -1 of row y + 1; the planes are at least TY + 1
+    // rows apart), never with a block cell.  BASE is cell (0, 0, 0).
+    static constexpr int BASE = HALO ? SXY + SX + 1 : 0;
+    static constexpr int NEL = HALO ? BASE + TZ * SXY + TY * SX + TX + 1 : TZ * SXY;
+    T v[NEL];
+    __device__ T& at(int x, int y, int z) { return v[BASE + z * SXY + y * SX + x]; }
 };
 
 template <bool FIX>
@@ -1080,7 +1088,20 @@ __device__ __forceinline__ int4 shadow_of_box(const VolDev& V, const DetDev& G, 
 // A block's shadows (and the views' sources in voxel units) are computed SHV
 // views at a time into LDS tables by all of the workgroup's lanes (8 per
 // view), instead of by every wavefront for every view.
-constexpr int SHV = 512;
+constexpr int SHV = 128;
+
+// A block's accumulator carries the one-cell halo (walks without range tests)
+// when it fits in the 160 KB of LDS beside the per-view tables (shadow,
+// source, divisor: 44 B per view) and the axis records.
+constexpr size_t LDS_BYTES = 160 * 1024;
+constexpr size_t BLOCK_TABLE_BYTES = SHV * (sizeof(int4) + 3 * sizeof(double) + sizeof(unsigned)) + 1024;
+template <int TX, int TY, int TZ, bool FIX>
+constexpr bool halo_fits()
+{
+    return sizeof(TileAcc<TX, TY, TZ, acc_t<FIX>, true>) + BLOCK_TABLE_BYTES <= LDS_BYTES;
+}
+template <int TX, int TY, int TZ, bool FIX>
+using BlockAcc = TileAcc<TX, TY, TZ, acc_t<FIX>, halo_fits<TX, TY, TZ, FIX>()>;
 
 
 // What a ray walking axis m needs of its block: the slab range on m, the
@@ -1095,20 +1116,25 @@ struct BlockAxis {
     float alo, ahi, blo, bhi;
     int Sa, Sb, St;
     int row0;        // LDS byte offset of slab 0's centred cell (0, 0): row = t St + row0
+    // the block's transverse range widened by the halo cell on each side:
+    // positions a with floor(a) in [la - 1, la + ea] are those in [hla, hha)
+    float hla, hha, hlb, hhb;
     int pad_[2];
 };
 
 template <bool REF, bool FIX, int TX, int TY, int TZ, int NW, int CS>
 __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __restrict__ pv, float* __restrict__ image,
-                                               int bx, int by, int bz, TileAcc<TX, TY, TZ, acc_t<FIX>>& acc,
+                                               int bx, int by, int bz, BlockAcc<TX, TY, TZ, FIX>& acc,
                                                int4* __restrict__ shadows, int es, bool accumulate)
 {
     using T = acc_t<FIX>;
-    constexpr int SX = TileAcc<TX, TY, TZ, T>::SX;
-    constexpr int SXY = TileAcc<TX, TY, TZ, T>::SXY;
+    using Acc = BlockAcc<TX, TY, TZ, FIX>;
+    constexpr int SX = Acc::SX;
+    constexpr int SXY = Acc::SXY;
+    constexpr bool HALO = Acc::HALO;
     constexpr int ES = (int)sizeof(T);
     const int tid = threadIdx.x;
-    T* af = &acc.v[0];
+    T* af = &acc.v[Acc::BASE];   // cell (0, 0, 0); halo cells lie at negative offsets and beyond TZ planes
     __syncthreads();   // the previous block's write-out has read acc
     for (int i = tid; i < TZ * SXY; i += 64 * NW) af[i] = (T)0;
     const int nx = P.V.n[0], ny = P.V.n[1], nz = P.V.n[2];
@@ -1141,6 +1167,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
         X.Sb = ES * ((m == 2) ? SX : SXY);
         X.St = ES * ((m == 0) ? 1 : ((m == 1) ? SX : SXY));
         X.row0 = -X.b0m * X.St - X.la * X.Sa - X.lb * X.Sb;
+        X.hla = (float)(X.la - 1);
+        X.hha = (float)(X.la + ext[a] + 1);
+        X.hlb = (float)(X.lb - 1);
+        X.hhb = (float)(X.lb + ext[b] + 1);
         X.pad_[0] = X.pad_[1] = 0;
         bax[m] = X;
     }
@@ -1297,18 +1327,33 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 const float pia = r.pa * r.inv_ka, pib = r.pb * r.inv_kb;
                 const int qs = FIX ? (int)sv : 0;               // |sv| <= 2^30
                 const int hi = (FIX && sv < 0.0f) ? -1 : 0;
-                auto walk = [&](auto step_tag, auto nonneg_tag) {
+                // the loop ends on the slab coordinate itself (exact small
+                // integers in FP32), two slabs per trip
+                const float tend = (float)(t1 - r.tc);
+                // The walk visits the positions a(t0), a(t0 + 1), ..., a(t1)
+                // (one FP32 fma of an exact slab coordinate each, monotone in
+                // t) and deposits into their floors (the crossing-stepped walk
+                // of near_unit_slope rays aside).  A ray whose end positions
+                // lie in the halo-widened range deposits only into block and
+                // halo cells; a wavefront of such rays walks without range
+                // tests, its out-of-block deposits landing in halo words that
+                // are never read.
+                bool in_halo = false;
+                if constexpr (HALO) {
+                    const float a_e = fmaf(r.ka, tend, r.Ac), b_e = fmaf(r.kb, tend, r.Bc);
+                    in_halo = fminf(a0, a_e) >= X.hla && fmaxf(a0, a_e) < X.hha && fminf(bb0, b_e) >= X.hlb &&
+                              fmaxf(bb0, b_e) < X.hhb;
+                }
+                auto walk = [&](auto step_tag, auto nonneg_tag, auto check_tag) {
                     constexpr bool STEP = decltype(step_tag)::value;
                     constexpr bool NONNEG = decltype(nonneg_tag)::value;
+                    constexpr bool CHECK = decltype(check_tag)::value;
                     float tw = tf;
                     int ca0 = cell_floor(a0), cb0 = cell_floor(bb0);
                     float fra0 = cell_frac(a0), frb0 = cell_frac(bb0);
                     int A0 = __mul24(ca0, Sa), B0 = __mul24(cb0, Sb);
                     int E = A0 + B0 + row;
-                    bool ia0 = (unsigned)(ca0 - la) < ea, ib0 = (unsigned)(cb0 - lb) < eb;
-                    // the loop ends on the slab coordinate itself (exact small
-                    // integers in FP32), two slabs per trip
-                    const float tend = (float)(t1 - r.tc);
+                    bool ia0 = !CHECK || (unsigned)(ca0 - la) < ea, ib0 = !CHECK || (unsigned)(cb0 - lb) < eb;
                     auto slab = [&]() {
                         tw += 1.0f;
                         const float a1 = fmaf(r.ka, tw, r.Ac), bb1 = fmaf(r.kb, tw, r.Bc);
@@ -1334,7 +1379,7 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                         const int A1 = __mul24(ca1, Sa), B1 = __mul24(cb1, Sb);
                         const int dA = A1 - A0, dB = B1 - B0;
                         const int X = E + dA + dB;
-                        const bool ia1 = (unsigned)(ca1 - la) < ea, ib1 = (unsigned)(cb1 - lb) < eb;
+                        const bool ia1 = !CHECK || (unsigned)(ca1 - la) < ea, ib1 = !CHECK || (unsigned)(cb1 - lb) < eb;
                         const bool i2 = (afirst & ia1 & ib0) | (!afirst & ia0 & ib1);
                         if constexpr (FIX) {
                             // the three pieces as differences of the truncated
@@ -1383,10 +1428,17 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                     }
                 };
                 auto walk_slope = [&](auto nonneg_tag) {
-                    if (near_unit_slope(r))
-                        walk(std::true_type{}, nonneg_tag);
-                    else
-                        walk(std::false_type{}, nonneg_tag);
+                    if (near_unit_slope(r)) {
+                        walk(std::true_type{}, nonneg_tag, std::true_type{});
+                    } else {
+                        if constexpr (HALO) {
+                            if (__all(in_halo)) {   // wave-uniform
+                                walk(std::false_type{}, nonneg_tag, std::false_type{});
+                                return;
+                            }
+                        }
+                        walk(std::false_type{}, nonneg_tag, std::true_type{});
+                    }
                 };
                 // wave-uniform: every walking ray of the wavefront non-negative
                 if (FIX && __all(hi == 0))
@@ -1447,7 +1499,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_siddon_adj_tile(Launch P, const 
                                                                 float* __restrict__ image,
                                                                 const unsigned* __restrict__ maxbits, float sbound)
 {
-    __shared__ TileAcc<TX, TY, TZ, acc_t<FIX>> acc;
+    __shared__ BlockAcc<TX, TY, TZ, FIX> acc;
     __shared__ int4 shadows[SHV];
     const int es = FIX ? fixed_exp(*maxbits, sbound) : 0;
     adj_tile_block<REF, FIX, TX, TY, TZ, NW, CS>(P, pv, image, blockIdx.x, blockIdx.y, blockIdx.z, acc, shadows, es,
@@ -1935,7 +1987,7 @@ template <bool REF, int CS>
 __global__ __launch_bounds__(64 * BNW) void k_batch_adj(const nefct_problem* __restrict__ probs, int nprob,
                                                         long long total, char* ws_base, unsigned flags)
 {
-    __shared__ TileAcc<BTX, BTY, BTZ, acc_t<true>> acc;
+    __shared__ BlockAcc<BTX, BTY, BTZ, true> acc;
     __shared__ int4 shadows[SHV];
     const long long t = (long long)blockIdx.x + 65536ll * (long long)blockIdx.y;
     if (t >= total) return;

@@ -259,8 +259,8 @@ def test_adjoint_algorithms_agree_with_fp64_transpose(oracle, algo):
 
 def test_adjoint_many_views(oracle):
     """1100 views: the tile adjoint fills its per-view LDS tables (detector
-    shadows, sources) 512 views at a time, so this crosses two chunk
-    boundaries.  Against the FP64 transpose (non-negative and mixed-sign
+    shadows, sources) 128 views at a time, so this crosses eight chunk
+    boundaries (the last chunk partial).  Against the FP64 transpose (non-negative and mixed-sign
     data, which take different deposit paths), the dot test, and the batched
     adjoint of the same problem."""
     nh = _nh()
@@ -278,8 +278,9 @@ def test_adjoint_many_views(oracle):
         got = nh.backproject(torch.tensor(y, device=DEV), vol, det, torch.tensor(ang)).cpu().numpy()
         ref = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
         assert _relerr(got, ref) < 1e-5, _relerr(got, ref)
-    x = torch.randn(vol.shape_zyx, device=DEV, dtype=torch.float32)
-    yt = torch.randn((nv, det.nb, det.na), device=DEV, dtype=torch.float32)
+    # seeded: with the global generator the draw depended on the tests run before
+    x = torch.randn(vol.shape_zyx, generator=torch.Generator().manual_seed(13)).to(DEV)
+    yt = torch.randn((nv, det.nb, det.na), generator=torch.Generator().manual_seed(14)).to(DEV)
     lhs = torch.sum(nh.project(x, vol, det, torch.tensor(ang)).double() * yt.double()).item()
     rhs = torch.sum(x.double() * nh.backproject(yt, vol, det, torch.tensor(ang)).double()).item()
     assert abs(lhs - rhs) <= 1e-6 * abs(lhs), (lhs, rhs)
@@ -331,6 +332,37 @@ def test_boundary_aligned_rays(oracle, shift):
     tol = 1e-5 if shift else 1e-2
     assert _relerr(got, ref) < tol, _relerr(got, ref)
     assert _relerr(base, ref_bp) < tol, _relerr(base, ref_bp)
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_fixed_point_adjoint_bitwise_across_block_shapes(oracle, signed):
+    """The fixed-point adjoint deposits the same integer pieces per (ray,
+    slab) whatever block owns the voxel, so every block shape gives the same
+    bits.  Blocks that fit a one-cell halo in LDS (32x32x16, 32x32x8, 32x16x8,
+    16x16x32) walk wavefronts of rays that stay within it without range
+    tests; 64x32x8 and 64x16x16 do not fit one and always test.  A volume
+    that is no multiple of any block side (ragged blocks), views near 45
+    degrees (transverse slopes near 1, where a ray can reach two cells past a
+    block and takes the tested walk) and a wide detector."""
+    nh = _nh()
+    from nefct_hip import ConeGeometry, VolumeGeometry, _lib
+    vol = VolumeGeometry((70, 45, 37), (2.0, -3.0, 1.0), (70 * 1.0, 45 * 1.0, 37 * 1.2))
+    det = ConeGeometry.flat(150.0, 260.0, 120, 1.6, 70, 1.5, ai=0.2, bi=0.4)
+    ang = np.concatenate([np.float32(math.pi / 4) + np.float32(0.004) * np.arange(-3, 4),
+                          (2 * math.pi * np.arange(9) / 9 + 0.1)]).astype(np.float32)
+    rng = np.random.default_rng(17 + int(signed))
+    y = (rng.standard_normal if signed else rng.uniform)(size=(ang.size, det.nb, det.na)).astype(np.float32)
+    yt = torch.tensor(y, device=DEV)
+    angt = torch.tensor(ang)
+    base = nh.backproject(yt, vol, det, angt, flags=_lib.F_TILE_64x32x8).cpu().numpy()
+    for fl in (_lib.F_TILE_32x32x16, _lib.F_TILE_32x32x16 | _lib.F_LANE_STRIDE2, _lib.F_TILE_32x32x8,
+               _lib.F_TILE_32x16x8, _lib.F_TILE_16x16x32, _lib.F_TILE_64x16x16):
+        got = nh.backproject(yt, vol, det, angt, flags=fl).cpu().numpy()
+        assert np.array_equal(got.view(np.int32), base.view(np.int32)), (hex(fl), _relerr(got, base))
+    ov = oracle.vol_struct(vol.grid, vol.center, vol.size)
+    od = oracle.det_struct(det.cyl, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi)
+    ref = oracle.exact_bp(ov, od, ang.astype(np.float64), y.astype(np.float64))
+    assert _relerr(base, ref) < 1e-5, _relerr(base, ref)
 
 
 def test_deterministic_adjoint_is_bitwise_reproducible():
