@@ -350,6 +350,8 @@ def fdk_leg(wl, img, sino, ang, reps=2):
     t_f, f = _timed(lambda: F.ramp_filter(q, "ram-lak"), reps)
     t_b, rec = _timed(lambda: F.backproject(f, vol, det, a), reps)
     t_bg, _ = _timed(lambda: F.backproject(f, vol, det, a, mapping="gather"), reps)
+    t_bs, _ = _timed(lambda: F.backproject(f, vol, det, a, mapping="staged"), reps)
+    t_bw, _ = _timed(lambda: F.backproject(f, vol, det, a, mapping="wide"), reps)
     nvox = vol.grid[0] * vol.grid[1] * vol.grid[2]
     # inner cylinder (radius 0.4 of the width), central half of the slices: no truncation or
     # cone-beam edge effects
@@ -368,8 +370,10 @@ def fdk_leg(wl, img, sino, ang, reps=2):
         "stage_ms": {"preweight": round(t_w * 1e3, 3), "filter": round(t_f * 1e3, 3),
                      "backproject": round(t_b * 1e3, 3)},
         "backproject_voxel_views_per_s": nvox * nv / t_b,
-        "backproject_mapping": "staged footprints (auto)",
+        "backproject_mapping": "staged footprints, 16x16x16 blocks (auto)",
         "backproject_gather_ms": round(t_bg * 1e3, 3),
+        "backproject_staged_ms": round(t_bs * 1e3, 3),
+        "backproject_wide_ms": round(t_bw * 1e3, 3),
         "backproject_hbm": {"algorithmic_bytes": bp_bytes, "achieved_GBs": round(bp_bytes / t_b / 1e9, 1),
                             "frac": round(bp_bytes / t_b / 1e9 / HBM_PEAK_GBS, 4)},
         "filter_gflops": round(2.0 * p.numel() * det.na / t_f / 1e9, 1),

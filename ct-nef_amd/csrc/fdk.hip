@@ -271,34 +271,35 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
     }
 }
 
-// Staged variant of k_fdk_bp.  A workgroup owns SB x SB voxel columns times SZ
-// voxels in z (a thread: one column's SZ voxels).  Per view it copies the
-// block's detector footprint into LDS once (the bounding rectangle of the
-// projected voxel-centre box with one cell of margin, ~1500 cells at
-// 512^3-class geometries) and reads every voxel's four taps from there,
-// instead of gathering 16 bytes per voxel and view through the vector cache
-// (~16 K taps per block and view).  The footprint of view iv + 1 is loaded
-// into registers before view iv is computed and written to the other LDS
-// buffer after it: one barrier per view.  Per voxel and view the arithmetic
-// is k_fdk_bp's, operation for operation, so both give identical images.  A
-// view whose footprint exceeds SRW x SRH cells is gathered from global
+// Staged variants of k_fdk_bp.  A workgroup owns BXS x BYS voxel columns
+// times SZ voxels in z (a thread: CPT columns' SZ voxels each).  Per view it
+// copies the block's detector footprint into LDS once (the bounding rectangle
+// of the projected voxel-centre box with one cell of margin) and reads every
+// voxel's four taps from there, instead of gathering 16 bytes per voxel and
+// view through the vector cache.  The footprint of view iv + 1 is loaded into
+// registers before view iv is computed and written to the other LDS buffer
+// after it: one barrier per view.  Per voxel and view the arithmetic is
+// k_fdk_bp's, operation for operation, so all mappings give identical images.
+// A view whose footprint exceeds RW x RH cells is gathered from global
 // memory, as k_fdk_bp does.  Footprints are bounded only when every voxel
 // centre lies in front of the source (U > 0 for every view), which the
 // launcher checks.
 //
-// LDS rows have a fixed stride of SRS floats: a voxel's four taps are two
+// Each detector cell of a view is read by every block its ray crosses, so the
+// bytes a view asks of the L2 fall with the block's side: 16 x 16 columns
+// (4 waves, 4 workgroups per CU: the default) or 64 x 64 columns (8 waves,
+// eight columns per thread, one workgroup and 130 KB of LDS per CU, blocks
+// dealt to the XCDs in contiguous ranges: a quarter of the footprint bytes
+// and a third of the HBM traffic, but slower, since one workgroup per CU
+// leaves nothing to run while its waves wait at the per-view barrier).
+//
+// LDS rows have a fixed stride of RW + 4 floats: a voxel's four taps are two
 // ds_read2_b32 at constant offsets (0, 1) and (SRS, SRS + 1) from one
-// address, and rows 4 floats apart in bank order spread lanes that read the
-// same column of different rows over different banks.  Staging: lane
-// (rsub, m) = (lane / 16, lane % 16) of wave w copies cells 4m .. 4m + 3 of
-// row 16 j + 4 w + rsub (j < SRH / 16) with one 16-byte load, whose address
-// is a per-view scalar plus a per-thread constant.
-constexpr int SB = 16;          // voxel columns per block side
+// address.  Staging: float4 slot j of thread t is float4 t + j T of the
+// footprint (T threads, RW / 4 float4 per row), one 16-byte load whose
+// address is a per-view scalar plus a per-thread constant; the lanes of a
+// wave cover whole rows, or runs of 16 float4 of 4 rows.
 constexpr int SZ = 16;          // voxels per thread along z
-constexpr int SRW = 64;         // staged footprint cells per row, at most
-constexpr int SRH = 64;         // staged footprint rows, at most
-constexpr int SRS = 68;         // LDS row stride (floats; a multiple of 4)
-constexpr int SNJ = SRH / 16;   // staging slots per thread and view
 
 struct Foot {
     int ua0, vb0, wq, H;        // first column and row, row length / 4, rows; wq == 0: not staged
@@ -309,11 +310,12 @@ struct Foot {
 // v0 + g (views past nv give an unstaged footprint), lane & 7 a corner.  The
 // projective map sends the box into the hull of its corners' images, so the
 // corners' extreme indices bound every voxel's taps; the one-cell margin
-// covers FP32 rounding of the voxels' row coordinates (~1e-3 cells).
+// covers FP32 rounding of the voxels' row coordinates (~1e-3 cells).  A
+// footprint wider than rw or taller than rh cells is not staged.
 __device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int v0,
                                double cx0, double cx1, double cy0, double cy1, double cz0, double cz1, double SID,
                                double SAD, double ha, double hb, double inv_da, double ai, double inv_db, double bi,
-                               int na, int nb)
+                               int na, int nb, int rw, int rh)
 {
     const int lane = threadIdx.x & 63, q = lane & 7;
     const int iv = min(v0 + (lane >> 3), nv - 1);
@@ -345,7 +347,7 @@ __device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __r
     const int ws = (ua1 - ua0 + 4) & ~3;
     ua0 = min(ua0, na - ws);
     const int H = vb1 - vb0 + 1;
-    const bool staged = v0 + (lane >> 3) < nv && ua0 >= 0 && ws <= SRW && H <= SRH;
+    const bool staged = v0 + (lane >> 3) < nv && ua0 >= 0 && ws <= rw && H <= rh;
     return make_int4(ua0, vb0, staged ? ws >> 2 : 0, H);
 }
 
@@ -368,25 +370,63 @@ __device__ __forceinline__ Foot foot_of(const int4& grp, int g)
     return f;
 }
 
-__global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restrict__ q, const double* __restrict__ trig,
-                                                       const float* __restrict__ offsets, int nv, int nx, int ny,
-                                                       int nz, double x0, double y0, double z0, double vx,
-                                                       double vy, double vz, double SID, double SAD, int na,
-                                                       double da, double ai, int nb, double db, double bi,
-                                                       float* __restrict__ image)
+template <int BXS, int BYS, int NWV, int RW, int RH>
+struct StagedShape {
+    static constexpr int T = 64 * NWV;                 // threads
+    static constexpr int CPT = BXS * BYS / T;          // voxel columns per thread
+    static constexpr int SRS = RW + 4;                 // LDS row stride (floats)
+    static constexpr int R4 = RW / 4;                  // float4 per footprint row
+    static constexpr int SNJ = RH * R4 / T;            // staging slots per thread and view
+    static_assert(CPT * T == BXS * BYS && SNJ * T == RH * R4 && RW % 4 == 0, "block shape");
+    static_assert(T % R4 == 0, "staging columns the same for every slot");
+};
+
+template <int BXS, int BYS, int NWV, int RW, int RH, bool XCDMAP>
+__global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_staged(
+    const float* __restrict__ q, const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int nx,
+    int ny, int nz, double x0, double y0, double z0, double vx, double vy, double vz, double SID, double SAD, int na,
+    double da, double ai, int nb, double db, double bi, float* __restrict__ image)
 {
-    __shared__ float4 lds4[2][SRH * SRS / 4];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int ix = blockIdx.x * SB + (lane & 15);
-    const int iy = blockIdx.y * SB + wave * 4 + (lane >> 4);
-    const int izc = blockIdx.z * SZ;
-    const bool active = ix < nx && iy < ny;
+    using S = StagedShape<BXS, BYS, NWV, RW, RH>;
+    constexpr int CPT = S::CPT, SRS = S::SRS, SNJ = S::SNJ, T = S::T, R4 = S::R4;
+    __shared__ float4 lds4[2][RH * SRS / 4];
+    const int tid = threadIdx.x;
+    // Block coordinates.  XCDMAP: a 1-D grid (a multiple of 8) whose
+    // workgroups the hardware deals to the 8 XCDs in turn (id mod 8) is
+    // remapped so that each XCD gets a contiguous eighth of the blocks (x
+    // fastest, then y, then z): the blocks an XCD runs at once are
+    // neighbours in one z slab, whose footprints share detector rows in
+    // that XCD's L2.
+    int bxi, byi, bzi;
+    if (XCDMAP) {
+        const int nbx = (nx + BXS - 1) / BXS, nby = (ny + BYS - 1) / BYS;
+        const int total = nbx * nby * ((nz + SZ - 1) / SZ);
+        const int per = (int)(gridDim.x >> 3);
+        const int n = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+        if (n >= total) return;                         // the whole workgroup: no barrier reached
+        bxi = n % nbx;
+        byi = (n / nbx) % nby;
+        bzi = n / (nbx * nby);
+    } else {
+        bxi = blockIdx.x;
+        byi = blockIdx.y;
+        bzi = blockIdx.z;
+    }
+    const int izc = bzi * SZ;
     const int kn = min(SZ, nz - izc);
-    const double x = x0 + (ix + 0.5) * vx, y = y0 + (iy + 0.5) * vy;
-    float acc[SZ];
+    // this thread's columns: column index tid + c T of the block, x fastest;
+    // T is a multiple of BXS, so all of them share x and step y by T / BXS
+    static_assert(T % BXS == 0, "columns of a thread share x");
+    constexpr int YSTEP = T / BXS;
+    const int ix = bxi * BXS + tid % BXS;
+    const int iy0 = byi * BYS + tid / BXS;
+    const double x = x0 + (ix + 0.5) * vx;
+    float acc[CPT][SZ];
 #pragma unroll
-    for (int k = 0; k < SZ; ++k) acc[k] = 0.0f;
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+        for (int k = 0; k < SZ; ++k) acc[c][k] = 0.0f;
+    }
     const float zc = (float)(z0 + (izc + 0.5) * vz), vzf = (float)vz;   // first voxel's z, step
     const double ha = na * da * 0.5, hb = nb * db * 0.5;
     const double inv_da = 1.0 / da, inv_db64 = 1.0 / db;
@@ -395,18 +435,23 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
     const float nb1 = (float)(nb - 1);
     const unsigned rowstep = 4u * (unsigned)na;
     // the block's voxel-centre box
-    const int bx1 = min((int)blockIdx.x * SB + SB - 1, nx - 1), by1 = min((int)blockIdx.y * SB + SB - 1, ny - 1);
-    const double cx0 = x0 + (blockIdx.x * SB + 0.5) * vx, cx1 = x0 + (bx1 + 0.5) * vx;
-    const double cy0 = y0 + (blockIdx.y * SB + 0.5) * vy, cy1 = y0 + (by1 + 0.5) * vy;
+    const int bx1 = min(bxi * BXS + BXS - 1, nx - 1), by1 = min(byi * BYS + BYS - 1, ny - 1);
+    const double cx0 = x0 + (bxi * BXS + 0.5) * vx, cx1 = x0 + (bx1 + 0.5) * vx;
+    const double cy0 = y0 + (byi * BYS + 0.5) * vy, cy1 = y0 + (by1 + 0.5) * vy;
     // (the box spans all SZ z slots, also those past the volume, whose samples
     // are dropped but whose clamped taps are read)
     const double cz0 = z0 + (izc + 0.5) * vz, cz1 = z0 + (izc + SZ - 1 + 0.5) * vz;
     auto footprints = [&](int v0) {
         return fdk_footprints(trig, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb, inv_da, ai,
-                              inv_db64, bi, na, nb);
+                              inv_db64, bi, na, nb, RW, RH);
     };
-    // this thread's staging cells: row 16 j + 4 wave + rsub, columns 4m .. 4m + 3
-    const int rsub = lane >> 4, m4 = 4 * (lane & 15);
+    // this thread's staging cells: float4 tid + j T of the footprint, row
+    // (tid + j T) / R4, cells 4 m .. 4 m + 3 with m = tid % R4 (T is a
+    // multiple of R4 or R4 of 64, so m is the same for every j)
+    const int m4 = 4 * (tid % R4);
+    const int rsub = tid / R4;                          // row of slot 0
+    constexpr int RSTEP = T / R4;                       // rows between slots (R4 <= T)
+    static_assert(R4 <= T, "one footprint row per slot set");
     const unsigned goff = 4u * (unsigned)(rsub * na + m4);    // bytes, within the view (< 2^31)
     const int soff = rsub * SRS + m4;                         // floats, within an LDS buffer
     auto stage_load = [&](const Foot& f, int jv, float4 (&st)[SNJ]) {
@@ -415,7 +460,7 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
         const char* qb = reinterpret_cast<const char*>(q + (size_t)nb * na * jv);
 #pragma unroll
         for (int j = 0; j < SNJ; ++j) {
-            const int r0 = 16 * j + 4 * wave;                 // wave-uniform
+            const int r0 = RSTEP * j;
             if (col_ok && r0 + rsub < f.H) st[j] = load16(qb + 4u * (unsigned)((f.vb0 + r0) * na + f.ua0) + goff);
         }
     };
@@ -425,7 +470,7 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
         float* lb = reinterpret_cast<float*>(&lds4[b][0]);
 #pragma unroll
         for (int j = 0; j < SNJ; ++j) {
-            const int r0 = 16 * j + 4 * wave;
+            const int r0 = RSTEP * j;
             if (col_ok && r0 + rsub < f.H) *reinterpret_cast<float4*>(lb + r0 * SRS + soff) = st[j];
         }
     };
@@ -449,13 +494,21 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
         // view iv, from LDS buffer iv & 1 (or from global memory)
         const float* qv = q + (size_t)nb * na * iv;
         const float off = offsets ? offsets[iv] : 0.0f;
-        const double c = trig[2 * iv], s = trig[2 * iv + 1];
-        const double U = SID + x * c + y * s;
-        if (active && U > 0.0) {
-            const double iU = rcp64(U);
-            const double mag = SAD * iU;
-            const double fa = ((-x * s + y * c) * mag + ha) * inv_da - ai;
-            if (fa >= 0.0 && fa <= (double)(na - 1)) {
+        const double cth = trig[2 * iv], sth = trig[2 * iv + 1];
+        // The next view's cells go to the other buffer (nobody reads it before
+        // the barrier below) once the first column is done: with several
+        // columns per thread their registers are then free for the others.
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            do {   // one column's SZ samples; break leaves it
+                const int iy = iy0 + c * YSTEP;
+                const double y = y0 + (iy + 0.5) * vy;
+                const double U = SID + x * cth + y * sth;
+                if (!(ix < nx && iy < ny && U > 0.0)) break;
+                const double iU = rcp64(U);
+                const double mag = SAD * iU;
+                const double fa = ((-x * sth + y * cth) * mag + ha) * inv_da - ai;
+                if (!(fa >= 0.0 && fa <= (double)(na - 1))) break;
                 const int ia0 = min((int)floor(fa), na - 2);
                 const float ta = (float)(fa - ia0);
                 const float wr = (float)(SID * iU);
@@ -478,8 +531,8 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
                         const float p00 = p[0], p01 = p[1], p10 = p[SRS], p11 = p[SRS + 1];
                         const float top = fmaf(ta, p01 - p00, p00);
                         const float bot = fmaf(ta, p11 - p10, p10);
-                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
-                        acc[k] = valid ? sum : acc[k];
+                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[c][k]);
+                        acc[c][k] = valid ? sum : acc[c][k];
                     }
                 } else {
                     const char* qb = reinterpret_cast<const char*>(qv);
@@ -496,22 +549,35 @@ __global__ __launch_bounds__(256, 4) void k_fdk_bp_staged(const float* __restric
                         __builtin_memcpy(&p1, qb + o1, 8);
                         const float top = fmaf(ta, p0.y - p0.x, p0.x);
                         const float bot = fmaf(ta, p1.y - p1.x, p1.x);
-                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[k]);
-                        acc[k] = valid ? sum : acc[k];
+                        const float sum = fmaf(w, fmaf(tb, bot - top, top), acc[c][k]);
+                        acc[c][k] = valid ? sum : acc[c][k];
+                        // several columns' accumulators leave no registers for
+                        // every sample's taps in flight at once
+                        if (CPT > 1 && k % 4 == 3) __builtin_amdgcn_sched_barrier(0);
                     }
                 }
-            }
+            } while (0);
+            if (c == 0) stage_store(nxt, jn & 1, st);
         }
-        stage_store(nxt, jn & 1, st);
         __syncthreads();
         cur = nxt;
     }
-    if (!active) return;
 #pragma unroll
-    for (int k = 0; k < SZ; ++k) {
-        if (k < kn) image[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * (size_t)(izc + k))] = acc[k];
+    for (int c = 0; c < CPT; ++c) {
+        const int iy = iy0 + c * YSTEP;
+        if (!(ix < nx && iy < ny)) continue;
+#pragma unroll
+        for (int k = 0; k < SZ; ++k) {
+            if (k < kn) image[(size_t)ix + (size_t)nx * ((size_t)iy + (size_t)ny * (size_t)(izc + k))] = acc[c][k];
+        }
     }
 }
+
+// the two staged shapes: 16 x 16 columns with footprints up to 64 x 64 cells,
+// 64 x 64 columns with footprints up to 256 x 64 cells
+constexpr int WBX = 64, WBY = 64;   // the wide blocks' columns
+#define NEFCT_FDK_SMALL 16, 16, 4, 64, 64, false
+#define NEFCT_FDK_WIDE WBX, WBY, 8, 256, 64, true
 
 __global__ void k_trig(const float* __restrict__ angles, int nv, double* __restrict__ trig)
 {
@@ -586,14 +652,15 @@ int nefct_fdk_backproject_mapped(const float* filtered, const int grid[3], const
         !finite_pos(size[1]) || !finite_pos(size[2]) || !std::isfinite(center[0]) || !std::isfinite(center[1]) ||
         !std::isfinite(center[2]) || !std::isfinite(ai) || !std::isfinite(bi))
         return NEFCT_ERR_GEOMETRY;
-    if (mapping != NEFCT_FDK_BP_AUTO && mapping != NEFCT_FDK_BP_GATHER && mapping != NEFCT_FDK_BP_STAGED)
+    if (mapping != NEFCT_FDK_BP_AUTO && mapping != NEFCT_FDK_BP_GATHER && mapping != NEFCT_FDK_BP_STAGED &&
+        mapping != NEFCT_FDK_BP_WIDE)
         return NEFCT_ERR_UNSUPPORTED;
     if (!workspace || workspace_bytes < nefct_fdk_workspace_size(nv)) return NEFCT_ERR_WORKSPACE;
     const int nx = grid[0], ny = grid[1], nz = grid[2];
     const double vx = (double)size[0] / nx, vy = (double)size[1] / ny, vz = (double)size[2] / nz;
     const double x0 = (double)center[0] - 0.5 * size[0], y0 = (double)center[1] - 0.5 * size[1],
                  z0 = (double)center[2] - 0.5 * size[2];
-    // The staged kernel needs bounded footprints: every voxel centre at least
+    // The staged kernels need bounded footprints: every voxel centre at least
     // 1e-3 SID in front of the source at every angle (U >= SID - r >= 1e-3 SID
     // with r the largest in-plane distance of a voxel centre from the axis).
     double r2 = 0.0;
@@ -602,16 +669,28 @@ int nefct_fdk_backproject_mapped(const float* filtered, const int grid[3], const
         r2 = std::max(r2, px * px + py * py);
     }
     const bool stageable = na >= 2 && nb >= 2 && std::sqrt(r2) <= 0.999 * (double)SID &&
-                           (ny + SB - 1) / SB <= 65535 && (nz + SZ - 1) / SZ <= 65535;
-    if (mapping == NEFCT_FDK_BP_STAGED && !stageable) return NEFCT_ERR_UNSUPPORTED;
+                           (ny + 15) / 16 <= 65535 && (nz + SZ - 1) / SZ <= 65535;
+    // the wide mapping's 1-D grid: block counts in int
+    const long long wide_blocks = (long long)((nx + WBX - 1) / WBX) * ((ny + WBY - 1) / WBY) * ((nz + SZ - 1) / SZ);
+    const bool wide_ok = wide_blocks <= (1ll << 30);
+    if ((mapping == NEFCT_FDK_BP_STAGED || mapping == NEFCT_FDK_BP_WIDE) && !stageable) return NEFCT_ERR_UNSUPPORTED;
+    if (mapping == NEFCT_FDK_BP_WIDE && !wide_ok) return NEFCT_ERR_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     double* trig = (double*)workspace;
     hipLaunchKernelGGL(k_trig, dim3((nv + 255) / 256), dim3(256), 0, s, angles, nv, trig);
     if (stageable && mapping != NEFCT_FDK_BP_GATHER) {
-        const dim3 g((nx + SB - 1) / SB, (ny + SB - 1) / SB, (nz + SZ - 1) / SZ);
-        hipLaunchKernelGGL(k_fdk_bp_staged, g, dim3(256), 0, s, filtered, (const double*)trig, offsets, nv, nx, ny,
-                           nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da, (double)ai, nb,
-                           (double)db, (double)bi, image);
+        if (mapping == NEFCT_FDK_BP_WIDE) {
+            // 1-D grid of every block, rounded up to a multiple of 8 (see XCDMAP)
+            const dim3 g((unsigned)((wide_blocks + 7) / 8 * 8));
+            hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_WIDE>), g, dim3(512), 0, s, filtered, (const double*)trig,
+                               offsets, nv, nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na,
+                               (double)da, (double)ai, nb, (double)db, (double)bi, image);
+        } else {
+            const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + SZ - 1) / SZ);
+            hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_SMALL>), g, dim3(256), 0, s, filtered, (const double*)trig,
+                               offsets, nv, nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na,
+                               (double)da, (double)ai, nb, (double)db, (double)bi, image);
+        }
         NEFCT_CHECK_LAUNCH();
         return NEFCT_OK;
     }

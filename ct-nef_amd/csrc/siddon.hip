@@ -1341,8 +1341,10 @@ __device__ __forceinline__ void adj_tile_block(const Launch& P, const float* __r
                 bool in_halo = false;
                 if constexpr (HALO) {
                     const float a_e = fmaf(r.ka, tend, r.Ac), b_e = fmaf(r.kb, tend, r.Bc);
-                    in_halo = fminf(a0, a_e) >= X.hla && fmaxf(a0, a_e) < X.hha && fminf(bb0, b_e) >= X.hlb &&
-                              fmaxf(bb0, b_e) < X.hhb;
+                    const float hla = X.hla, hha = X.hha, hlb = X.hlb, hhb = X.hhb;
+                    // bitwise and: all four tests at once, no branch per test
+                    in_halo = (fminf(a0, a_e) >= hla) & (fmaxf(a0, a_e) < hha) & (fminf(bb0, b_e) >= hlb) &
+                              (fmaxf(bb0, b_e) < hhb);
                 }
                 auto walk = [&](auto step_tag, auto nonneg_tag, auto check_tag) {
                     constexpr bool STEP = decltype(step_tag)::value;

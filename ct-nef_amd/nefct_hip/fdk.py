@@ -107,13 +107,16 @@ def ramp_filter(q: torch.Tensor, window: str = "ram-lak") -> torch.Tensor:
     return out
 
 
-MAPPINGS = {"auto": 0, "gather": 1, "staged": 2}
+MAPPINGS = {"auto": 0, "gather": 1, "staged": 2, "wide": 3}
 
 
 def backproject(filtered: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, angles,
-                offsets=None, mapping: str = "auto") -> torch.Tensor:
+                offsets=None, mapping: str = "auto", out: torch.Tensor = None) -> torch.Tensor:
     """Voxel-driven weighted backprojection.  ``mapping`` picks the work
-    mapping ("auto", "gather" or "staged"; identical images)."""
+    mapping ("auto", "gather", "staged" (16x16x16 blocks) or "wide"
+    (64x64x16 blocks); identical images).  ``out``: an existing
+    [nz, ny, nx] float32 tensor on the projections' device to write every
+    voxel of (else a new one)."""
     if mapping not in MAPPINGS:
         raise _lib.NefctError(f"unknown mapping {mapping!r}; one of {sorted(MAPPINGS)}")
     _need_device(filtered)
@@ -134,7 +137,13 @@ def backproject(filtered: torch.Tensor, vol: VolumeGeometry, det: ConeGeometry, 
     wsb = int(L.nefct_fdk_workspace_size(nv))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     nx, ny, nz = vol.grid
-    img = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    if out is None:
+        img = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    else:
+        if (tuple(out.shape) != (nz, ny, nx) or out.dtype != torch.float32 or out.device != dev
+                or not out.is_contiguous()):
+            raise _lib.NefctError(f"out must be a contiguous float32 [{nz}, {ny}, {nx}] tensor on {dev}")
+        img = out
     st = L.nefct_fdk_backproject_mapped(_ptr(filtered), _lib.int3(vol.grid), _lib.float3(vol.center),
                                         _lib.float3(vol.size), _ptr(ang), _ptr(off) if off is not None else None,
                                         nv, det.SID, det.SAD, det.na, det.da, det.ai, det.nb, det.db, det.bi,

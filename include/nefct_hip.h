@@ -334,15 +334,18 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
                           size_t workspace_bytes, void* stream);
 
 /* nefct_fdk_backproject with a chosen work mapping (identical images):
- * NEFCT_FDK_BP_AUTO the staged mapping where eligible, else gathers (what
- * nefct_fdk_backproject does); NEFCT_FDK_BP_GATHER every voxel's taps gathered
+ * NEFCT_FDK_BP_AUTO the 16x16x16 staged mapping where eligible, else gathers
+ * (what nefct_fdk_backproject does); NEFCT_FDK_BP_GATHER every voxel's taps gathered
  * from global memory; NEFCT_FDK_BP_STAGED each 16x16x16-voxel block's detector
- * footprint staged in LDS once per view (NEFCT_ERR_UNSUPPORTED unless
- * na, nb >= 2 and every voxel centre lies within 0.999 SID of the rotation
- * axis in-plane).  No reference counterpart (a tuning choice). */
+ * footprint staged in LDS once per view; NEFCT_FDK_BP_WIDE the same for
+ * 64x64x16-voxel blocks (a third of the HBM traffic, slower).  The staged
+ * mappings return NEFCT_ERR_UNSUPPORTED unless na, nb >= 2 and every voxel
+ * centre lies within 0.999 SID of the rotation axis in-plane.  No reference
+ * counterpart (a tuning choice). */
 #define NEFCT_FDK_BP_AUTO 0
 #define NEFCT_FDK_BP_GATHER 1
 #define NEFCT_FDK_BP_STAGED 2
+#define NEFCT_FDK_BP_WIDE 3
 int nefct_fdk_backproject_mapped(const float* filtered, const int grid[3], const float center[3],
                                  const float size[3], const float* angles, const float* offsets, int nv, float SID,
                                  float SAD, int na, float da, float ai, int nb, float db, float bi, float* image,

@@ -1,7 +1,7 @@
 """The FDK backprojection's work mappings on the GPU (C ABI
 nefct_fdk_backproject_mapped): per-voxel gathers and LDS-staged block
-footprints run the same per-voxel arithmetic in the same view order, so their
-images are bitwise identical.  Checked at the C2 geometry (512^3-class volume,
+footprints (16x16x16 and 64x64x16 blocks) run the same per-voxel arithmetic in
+the same view order, so their images are bitwise identical.  Checked at the C2 geometry (512^3-class volume,
 1024x768 detector) on a view subset, with helical offsets, on ragged shapes
 with a shifted volume and odd detector widths, on a fine detector whose block
 footprints overflow the LDS buffer (the per-view fallback to gathers), and
@@ -21,12 +21,25 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def _both(q, vol, det, ang, offsets=None):
+def _run(q, vol, det, ang, offsets, mapping):
+    """One backprojection into a NaN-filled image: a voxel the mapping does not
+    write stays NaN (a fresh tensor may hold an earlier call's image)."""
     from nefct_hip import fdk as F
-    g = F.backproject(q, vol, det, ang, offsets=offsets, mapping="gather")
-    s = F.backproject(q, vol, det, ang, offsets=offsets, mapping="staged")
-    a = F.backproject(q, vol, det, ang, offsets=offsets)
+    nx, ny, nz = vol.grid
+    out = torch.full((nz, ny, nx), float("nan"), device=DEV)
+    return F.backproject(q, vol, det, ang, offsets=offsets, mapping=mapping, out=out)
+
+
+def _both(q, vol, det, ang, offsets=None):
+    """gather, staged (small blocks) and automatic images; the wide-block
+    image is checked against the gathered one here as well."""
+    g = _run(q, vol, det, ang, offsets, "gather")
+    s = _run(q, vol, det, ang, offsets, "staged")
+    w = _run(q, vol, det, ang, offsets, "wide")
+    a = _run(q, vol, det, ang, offsets, "auto")
     torch.cuda.synchronize()
+    assert not torch.isnan(g).any(), "gathered image not fully written"
+    assert torch.equal(g, w), "wide-block mapping differs from the gathered image"
     return g, s, a
 
 
@@ -90,7 +103,7 @@ def test_footprint_overflow_falls_back_bitwise():
     assert torch.equal(g, s) and torch.equal(g, a)
 
 
-@pytest.mark.parametrize("mapping", ["gather", "staged"])
+@pytest.mark.parametrize("mapping", ["gather", "staged", "wide"])
 def test_each_mapping_vs_oracle(oracle, mapping):
     from nefct_hip import ConeGeometry, VolumeGeometry
     from nefct_hip import fdk as F
@@ -102,7 +115,7 @@ def test_each_mapping_vs_oracle(oracle, mapping):
     rng = np.random.default_rng(11)
     q = rng.standard_normal((nv, det.nb, det.na)).astype(np.float32)
     ang = (2 * np.pi * np.arange(nv) / nv + 0.2).astype(np.float32)
-    bp = F.backproject(torch.tensor(q, device=DEV), vol, det, torch.tensor(ang), mapping=mapping).cpu().numpy()
+    bp = _run(torch.tensor(q, device=DEV), vol, det, torch.tensor(ang), None, mapping).cpu().numpy()
     want = oracle.fdk_backproject(ov, od, ang.astype(np.float64), q.astype(np.float64))
     assert np.count_nonzero(want) > 0
     assert _rel(bp, want) < 1e-5
@@ -116,8 +129,9 @@ def test_volume_behind_source_refused_by_staged():
     nv = 6
     q = torch.ones((nv, det.nb, det.na), device=DEV)
     ang = torch.tensor((2 * np.pi * np.arange(nv) / nv).astype(np.float32))
-    with pytest.raises(_lib.NefctError):
-        F.backproject(q, vol, det, ang, mapping="staged")
+    for m in ("staged", "wide"):
+        with pytest.raises(_lib.NefctError):
+            F.backproject(q, vol, det, ang, mapping=m)
     g = F.backproject(q, vol, det, ang, mapping="gather")
     a = F.backproject(q, vol, det, ang)
     assert torch.equal(g, a)
