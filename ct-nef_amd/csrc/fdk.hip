@@ -447,7 +447,7 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
     };
     // this thread's staging cells: float4 tid + j T of the footprint, row
     // (tid + j T) / R4, cells 4 m .. 4 m + 3 with m = tid % R4 (T is a
-    // multiple of R4 or R4 of 64, so m is the same for every j)
+    // multiple of R4, so m is the same for every j)
     const int m4 = 4 * (tid % R4);
     const int rsub = tid / R4;                          // row of slot 0
     constexpr int RSTEP = T / R4;                       // rows between slots (R4 <= T)
