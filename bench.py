@@ -422,7 +422,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default launch shapes instead of the autotuner's (tuning is untimed)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain and C4 legs")
-    ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched-vs-plain legs (64 uneven problems and C4)")
     ap.add_argument("--batched-iters", type=int, default=2)
     ap.add_argument("--no-fdk", action="store_true", help="skip the FDK and warp legs")
     ap.add_argument("--calib", action="store_true",
@@ -444,7 +444,7 @@ def main():
     vol, det = wl.vol, wl.det
     dev = torch.device("cuda:0")
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(dev, args.batched_iters)}), flush=True)
+        print(json.dumps({"batched": batched_leg(dev, args.batched_iters), "c4": c4_leg(dev)}), flush=True)
         return
     if args.calib:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
