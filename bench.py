@@ -383,6 +383,44 @@ def fdk_leg(wl, img, sino, ang, reps=2):
     }
 
 
+def backprojector_leg(wl, sino, ang, reps=2):
+    """The reference's two other backprojectors at the workload's size: the
+    voxel-driven one (nefct/cc/pixel_3d/pixel.cu.cc, four taps per voxel and
+    view) over the FDK leg's short scan, and the pixel-driven FDK footprint one
+    (nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc, float atomics into the
+    volume) over the first 8 views, in that file's voxel units."""
+    import torch
+    from nefct_hip.fdk import footprint_backproject
+    from nefct_hip.pixel import pixel_backproject
+    vol, det = wl.vol, wl.det
+    nv_full = ang.numel()
+    gamma = math.atan(0.5 * det.na * det.da / det.SAD)
+    nv = min(nv_full, int(math.ceil((math.pi + 2.0 * gamma) / (2.0 * math.pi / nv_full))) + 1)
+    p = sino[:nv].contiguous()
+    a = ang[:nv].contiguous()
+    nvox = vol.grid[0] * vol.grid[1] * vol.grid[2]
+    t_p, _ = _timed(lambda: pixel_backproject(p, vol, det, a), reps)
+    b_p = 4 * p.numel() + 4 * nvox
+    nf = 8
+    vx = vol.voxel[0]
+    pf = sino[:nf].contiguous()
+    af = ang[:nf].contiguous()
+    out = torch.zeros(vol.shape_zyx, dtype=torch.float32, device=sino.device)
+    # the footprint op's naming: SAD = source to axis, SID = source to detector
+    kw = dict(SID=det.SAD / vx, SAD=det.SID / vx, da=det.da / vx, ai=0.0, na=det.na, db=det.db / vx, bi=0.0,
+              nb=det.nb)
+    t_f, _ = _timed(lambda: footprint_backproject(pf, vol.grid, af, out=out, **kw), reps)
+    log(f"[bench] backprojectors: voxel-driven {t_p * 1e3:.2f} ms ({nv} views), footprint {t_f * 1e3:.2f} ms "
+        f"({nf} views)")
+    return {
+        "voxel_driven": {"views": nv, "ms": round(t_p * 1e3, 3), "voxel_views_per_s": nvox * nv / t_p,
+                         "algorithmic_bytes": b_p, "achieved_GBs": round(b_p / t_p / 1e9, 1),
+                         "frac": round(b_p / t_p / 1e9 / HBM_PEAK_GBS, 4)},
+        "fdk_footprint": {"views": nf, "ms": round(t_f * 1e3, 3), "ms_per_view": round(t_f * 1e3 / nf, 3),
+                          "rays_per_s": nf * det.na * det.nb / t_f},
+    }
+
+
 def warp_leg(wl, img, reps=3):
     """Trilinear warp of the workload's volume by a smooth displacement field
     (forward, transpose scatter, displacement gradient), HBM rate of each."""
@@ -523,10 +561,11 @@ def main():
     b_dom = bytes_adj if t_adj > t_fwd else bytes_fwd
     achieved = b_dom / t_dom / 1e9
 
-    fdk = warp_rates = None
+    fdk = warp_rates = bps = None
     if not args.no_fdk and not args.views and args.only == "pair":
         sino_fdk = nh.project(img, vol, det, ang)
         fdk = fdk_leg(wl, img, sino_fdk, ang)
+        bps = backprojector_leg(wl, sino_fdk, ang)
         del sino_fdk
         warp_rates = warp_leg(wl, img)
         torch.cuda.empty_cache()
@@ -636,6 +675,7 @@ def main():
         "batched": batched,
         "c4": c4,
         "fdk": fdk,
+        "backprojectors": bps,
         "warp": warp_rates,
         "cpu_baseline": cpu,
     }
