@@ -87,6 +87,54 @@ def test_footprint_backproject_vs_oracle(oracle, cyl, helical):
     assert _rel(got.cpu().numpy(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("geom", ["fine", "coarse"])
+def test_footprint_backproject_lds_window_paths(oracle, geom):
+    """The kernel pre-sums each workgroup's footprints over chunks of slabs in
+    an LDS window.  "fine": C2's proportions in voxel units (detector column
+    0.5 voxel at the axis), views at and around 45 deg so workgroups whose
+    pixels disagree on the walk axis add directly, most chunks fit the window;
+    "coarse": 2-voxel columns, most chunks exceed it and add directly."""
+    from nefct_hip.fdk import footprint_backproject
+    if geom == "fine":
+        nx, ny, nz, na, nb = 96, 96, 48, 192, 144
+        SID, SAD, da, ai, db, bi = 375.0, 250.0, 0.75, 0.0, 0.75, 0.0
+        ang = np.deg2rad([0.0, 20.0, 40.0, 45.0, 50.0, 100.0, 225.0]).astype(np.float32)
+    else:
+        nx, ny, nz, na, nb = 48, 40, 32, 48, 24
+        SID, SAD, da, ai, db, bi = 150.0, 100.0, 3.0, 0.2, 2.5, 0.1
+        ang = np.deg2rad([0.0, 30.0, 45.0, 77.0, 190.0]).astype(np.float32)
+    nv = ang.size
+    pv = _smooth_sino(nv, nb, na, 17)
+    pv[:, :, ::7] = 0.0                      # zero pixels add nothing (and are skipped)
+    got = footprint_backproject(torch.tensor(pv, device=DEV), (nx, ny, nz), torch.tensor(ang, device=DEV),
+                                None, SID=SID, SAD=SAD, da=da, ai=ai, na=na, db=db, bi=bi, nb=nb)
+    ref = oracle.fdk_fp_bp(0, (nx, ny, nz), ang.astype(np.float64), None, SID, SAD, na, da, ai, nb, db, bi,
+                           pv.astype(np.float64))
+    assert ref.std() > 0
+    assert _rel(got.cpu().numpy(), ref) < 1e-5
+
+
+def test_footprint_backproject_full_size_vs_oracle(oracle):
+    """512 x 512 x 256 voxels, 1024 x 768 detector of 0.75-voxel pixels,
+    source 1250 and detector 1875 voxels from the source-axis line: two views
+    (one within 3 deg of a diagonal, where workgroups straddle the walk-axis
+    switch) against the FP64 oracle.  Bar 5e-5: at these distances the FP32
+    edge rays (as the reference computes them) place footprint edges to ~1e-4
+    voxel, which moves boundary-cell weights by ~2e-4; the full-size error is
+    ~2.4e-5."""
+    from nefct_hip.fdk import footprint_backproject
+    nx, ny, nz, na, nb = 512, 512, 256, 1024, 768
+    kw = dict(SID=1875.0, SAD=1250.0, da=0.75, ai=0.0, na=na, db=0.75, bi=0.0, nb=nb)
+    ang = np.array([0.3, 0.83], np.float32)
+    gen = torch.Generator(device=DEV).manual_seed(1)
+    pv = torch.rand((2, nb, na), generator=gen, device=DEV)
+    got = footprint_backproject(pv, (nx, ny, nz), torch.tensor(ang, device=DEV), **kw).cpu().numpy()
+    ref = oracle.fdk_fp_bp(0, (nx, ny, nz), ang.astype(np.float64), None, 1875.0, 1250.0, na, 0.75, 0.0, nb, 0.75,
+                           0.0, pv.cpu().numpy().astype(np.float64))
+    assert ref.std() > 0
+    assert _rel(got, ref) < 5e-5
+
+
 def test_footprint_backproject_accumulates_into_out():
     from nefct_hip.fdk import footprint_backproject
     pv = torch.rand((3, 8, 16), device=DEV)
