@@ -19,9 +19,16 @@
 // 64 consecutive x, each lane keeps PZ z-values in registers, so the
 // per-view projection (x', y', magnification, a) is computed once per column
 // and reused for PZ voxels.  cos/sin of a chunk of views are staged in LDS
-// by the workgroup (computed in FP64 and rounded to FP32).  The bilinear taps
-// of neighbouring lanes are neighbouring detector columns of the same view,
-// so the gathers coalesce.
+// by the workgroup (computed in FP64 and rounded to FP32).  Per voxel the row
+// coordinate is one FMA, b = z * (t / db) + (nb/2 - bi - 0.5), instead of
+// the reference's product, division and two subtractions (the same value to
+// an ulp or two of b), and the four taps are two 8-byte loads (the skip rule
+// keeps the 2 x 2 neighbourhood on the detector, so ia <= na - 2 and
+// ib <= nb - 2), issued for all PZ voxels before their arithmetic; a voxel
+// whose neighbourhood leaves the detector is clamped for the address and its
+// sample dropped by a select.  The bilinear sample is formed as two lerps.
+// The taps of neighbouring lanes are neighbouring detector columns of the
+// same view, so the gathers coalesce.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,7 +38,7 @@
 
 namespace {
 
-constexpr int PZ = 8;      // z-values per lane
+constexpr int PZ = 16;     // z-values per lane
 constexpr int PV = 256;    // views staged in LDS per chunk
 
 template <bool CYL>
@@ -46,6 +53,7 @@ __global__ __launch_bounds__(256) void k_pixel_bp(const float* __restrict__ pv, 
     const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int iz0 = blockIdx.z * PZ;
     const bool live = ix < nx && iy < ny;
+    const int kn = min(PZ, nz - iz0);
     // voxel centres, formed in double and rounded as the reference does
     const float xc = (float)((ix + 0.5) * sx / nx + cx);
     const float yc = (float)((iy + 0.5) * sy / ny + cy);
@@ -56,6 +64,9 @@ __global__ __launch_bounds__(256) void k_pixel_bp(const float* __restrict__ pv, 
 #pragma unroll
     for (int k = 0; k < PZ; ++k) acc[k] = 0.0f;
     const float ha = (float)(na / 2), hb = (float)(nb / 2);
+    const float cb = hb - bi - 0.5f, nb1 = (float)(nb - 1);
+    const unsigned rowstep = 4u * (unsigned)na;
+    const unsigned row1 = nb >= 2 ? rowstep : 0u;   // one-row detectors: every voxel is skipped
     for (int v0 = 0; v0 < nv; v0 += PV) {
         const int nvc = min(PV, nv - v0);
         __syncthreads();
@@ -82,19 +93,29 @@ __global__ __launch_bounds__(256) void k_pixel_bp(const float* __restrict__ pv, 
             if (!(a >= 0.0f && a < (float)(na - 1))) continue;
             const float fa = floorf(a);
             const int ia = (int)fa;
-            const float wa2 = a - fa, wa = 1.0f - wa2;
-            const float* pview = pv + (size_t)(v0 + j) * na * nb + ia;
+            const float wa2 = a - fa;
+            const float tdb = t / db;
+            const char* pview = reinterpret_cast<const char*>(pv + (size_t)(v0 + j) * na * nb) + 4u * (unsigned)ia;
+            float2 p0[PZ], p1[PZ];
+            float wb2[PZ];
+            bool valid[PZ];
 #pragma unroll
             for (int k = 0; k < PZ; ++k) {
-                const float zd = t * zc[k];
-                const float b = zd / db + hb - bi - 0.5f;
-                if (!(b >= 0.0f && b < (float)(nb - 1))) continue;
-                const float fb = floorf(b);
-                const int ib = (int)fb;
-                const float wb2 = b - fb, wb = 1.0f - wb2;
-                const float* r0 = pview + (size_t)ib * na;
-                const float* r1 = r0 + na;
-                acc[k] += r0[0] * wa * wb + r1[0] * wa * wb2 + r0[1] * wa2 * wb + r1[1] * wa2 * wb2;
+                const float b = fmaf(zc[k], tdb, cb);
+                valid[k] = k < kn && b >= 0.0f && b < nb1;   // (pixel.cu.cc:47-50)
+                const float bcl = __builtin_amdgcn_fmed3f(b, 0.0f, nb1);
+                const int ib = min((int)bcl, max(nb - 2, 0));   // bcl >= 0: truncation is floor
+                wb2[k] = bcl - (float)ib;
+                const unsigned o0 = __umul24((unsigned)ib, rowstep);
+                __builtin_memcpy(&p0[k], pview + o0, 8);
+                __builtin_memcpy(&p1[k], pview + o0 + row1, 8);
+            }
+#pragma unroll
+            for (int k = 0; k < PZ; ++k) {
+                const float top = fmaf(wa2, p0[k].y - p0[k].x, p0[k].x);
+                const float bot = fmaf(wa2, p1[k].y - p1[k].x, p1[k].x);
+                const float sum = acc[k] + fmaf(wb2[k], bot - top, top);
+                acc[k] = valid[k] ? sum : acc[k];
             }
         }
     }
@@ -114,6 +135,8 @@ int run(bool cyl, const float* pv, const int grid[3], const float center[3], con
     const int nx = grid[0], ny = grid[1], nz = grid[2];
     if (nx <= 0 || ny <= 0 || nz <= 0 || nv <= 0 || na <= 0 || nb <= 0) return NEFCT_ERR_SHAPE;
     if ((ny + 3) / 4 > 65535 || (nz + PZ - 1) / PZ > 65535) return NEFCT_ERR_SHAPE;
+    // in-view byte offsets are 24-bit products (row * 4 na) and 32-bit sums
+    if (4ull * na >= (1ull << 24) || nb >= (1 << 24) || 4ull * na * nb >= (1ull << 32)) return NEFCT_ERR_SHAPE;
     for (float v : {center[0], center[1], center[2], size[0], size[1], size[2], SID, SAD, da, ai, db, bi})
         if (!std::isfinite(v)) return NEFCT_ERR_GEOMETRY;
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0 && SAD > 0))
