@@ -6,6 +6,8 @@
   magnitude (its scale 2^e then lies beyond the FP32 range).
 * Shape mismatches between projections, detector, angles and offsets, and
   between warp inputs, raise before anything is launched.
+* The voxel-driven backprojection on one-row / one-column detectors (every
+  view skipped) and beyond its offset limits.
 """
 import math
 
@@ -81,3 +83,29 @@ def test_deform_shape_checks():
         deform_dvf_grad_op(a, b, a, a, a, 1)
     with pytest.raises(NefctError):
         deform_op(a[None], a[None], a[None], a[None], 1)
+
+
+def test_pixel_backproject_degenerate_detectors_and_limits():
+    """One-row and one-column detectors skip every view (the 2 x 2 tap
+    neighbourhood never fits), so the image is zero and no tap is read past the
+    projection; a detector beyond the 24-bit row offsets is refused."""
+    from nefct_hip import _lib
+    from nefct_hip.pixel import pixel_backproject_op
+    grid, low, size = [12, 10, 8], [-6.0, -5.0, -4.0], [12.0, 10.0, 8.0]
+    ang = torch.tensor([0.0, 0.7, 2.0], device=DEV)
+    for na, nb in ((16, 1), (1, 16)):
+        pv = torch.rand((3, nb, na), device=DEV) + 1.0
+        out = pixel_backproject_op(pv, grid, low, size, ang, 100.0, 150.0, na, 1.0, 0.0, nb, 1.0, 0.0, 0)
+        torch.cuda.synchronize()
+        assert out.abs().max().item() == 0.0, (na, nb)
+    pv = torch.rand((3, 4, 4), device=DEV)
+    img = torch.zeros((8, 10, 12), device=DEV)
+    st = _lib.load().nefct_pixel_backproject_flat(pv.data_ptr(), _lib.int3(grid), _lib.float3(low),
+                                                  _lib.float3(size), ang.data_ptr(), 3, 100.0, 150.0, 1 << 22, 1.0,
+                                                  0.0, 4, 1.0, 0.0, img.data_ptr(), None)
+    assert st != 0
+    st = _lib.load().nefct_pixel_backproject_flat(pv.data_ptr(), _lib.int3(grid), _lib.float3(low),
+                                                  _lib.float3(size), ang.data_ptr(), 3, 100.0, 150.0, 4, 1.0,
+                                                  0.0, 4, 1.0, 0.0, img.data_ptr(), None)
+    torch.cuda.synchronize()
+    assert st == 0
