@@ -378,7 +378,10 @@ int nefct_pixel_backproject_cyli(const float* projection_value, const int grid[3
  * (cylindrical).  shape = (nx, ny, nz) on the host (the reference copies it
  * from the device); offsets[nv] (may be NULL = 0) lift source and detector
  * along z.  Adds into image [nz][ny][nx], as the reference does (the caller
- * zeroes it first); additions are atomic, so the summation order varies. */
+ * zeroes it first); additions are atomic, so the summation order varies.
+ * A workgroup (64 columns x 4 rows of one view) first sums its additions over
+ * a few slabs in 64-bit fixed point whose step is 2^-30 of its largest
+ * possible contribution (max |pv| |r|/|r_dom| SAD^2 / U_min^2). */
 int nefct_fdk_footprint_backproject_flat(const float* pv_values, const int shape[3], const float* offsets,
                                          const float* angles, int nv, float SID, float SAD, float da, float ai,
                                          int na, float db, float bi, int nb, float* image, void* stream);
