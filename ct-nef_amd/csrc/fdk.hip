@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "fdk_staged.h"
 
 namespace {
 
@@ -312,14 +313,33 @@ struct Foot {
 // corners' extreme indices bound every voxel's taps; the one-cell margin
 // covers FP32 rounding of the voxels' row coordinates (~1e-3 cells).  A
 // footprint wider than rw or taller than rh cells is not staged.
-__device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int v0,
-                               double cx0, double cx1, double cy0, double cy1, double cz0, double cz1, double SID,
-                               double SAD, double ha, double hb, double inv_da, double ai, double inv_db, double bi,
-                               int na, int nb, int rw, int rh)
+// lane l's value of a double held per lane
+__device__ __forceinline__ double read_lane64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <bool PIX>
+__device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __restrict__ angles,
+                               const float* __restrict__ offsets, int nv, int v0, double cx0, double cx1,
+                               double cy0, double cy1, double cz0, double cz1, double SID, double SAD, double ha,
+                               double hb, double inv_da, double ai, double inv_db, double bi, int na, int nb, int rw,
+                               int rh)
 {
     const int lane = threadIdx.x & 63, q = lane & 7;
     const int iv = min(v0 + (lane >> 3), nv - 1);
-    const double c = trig[2 * iv], s = trig[2 * iv + 1];
+    double c, s;
+    if (PIX) {
+        const double a = (double)angles[iv];
+        c = cos(a);
+        s = sin(a);
+    } else {
+        c = trig[2 * iv];
+        s = trig[2 * iv + 1];
+    }
     const double off = offsets ? (double)offsets[iv] : 0.0;
     const double x = (q & 1) ? cx1 : cx0, y = (q & 2) ? cy1 : cy0, z = (q & 4) ? cz1 : cz0;
     const double U = SID + x * c + y * s;
@@ -381,11 +401,12 @@ struct StagedShape {
     static_assert(T % R4 == 0, "staging columns the same for every slot");
 };
 
-template <int BXS, int BYS, int NWV, int RW, int RH, bool XCDMAP>
+template <int BXS, int BYS, int NWV, int RW, int RH, bool XCDMAP, bool PIX = false>
 __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_staged(
     const float* __restrict__ q, const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int nx,
     int ny, int nz, double x0, double y0, double z0, double vx, double vy, double vz, double SID, double SAD, int na,
-    double da, double ai, int nb, double db, double bi, float* __restrict__ image)
+    double da, double ai, int nb, double db, double bi, float* __restrict__ image,
+    const float* __restrict__ angles = nullptr)
 {
     using S = StagedShape<BXS, BYS, NWV, RW, RH>;
     constexpr int CPT = S::CPT, SRS = S::SRS, SNJ = S::SNJ, T = S::T, R4 = S::R4;
@@ -428,12 +449,31 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
         for (int k = 0; k < SZ; ++k) acc[c][k] = 0.0f;
     }
     const float zc = (float)(z0 + (izc + 0.5) * vz), vzf = (float)vz;   // first voxel's z, step
-    const double ha = na * da * 0.5, hb = nb * db * 0.5;
+    // PIX: the voxel-driven op's detector indices, y' mag / da + na/2 (integer
+    // na/2) - ai (the launcher passes ai + 0.5, bi + 0.5)
+    const double ha = PIX ? (double)(na / 2) * da : na * da * 0.5;
+    const double hb = PIX ? (double)(nb / 2) * db : nb * db * 0.5;
     const double inv_da = 1.0 / da, inv_db64 = 1.0 / db;
     const float inv_db = (float)inv_db64;
     const float fb0 = (float)(hb / db - bi);
     const float nb1 = (float)(nb - 1);
     const unsigned rowstep = 4u * (unsigned)na;
+    // PIX: cos / sin of views v0 + lane of the current 64-view group, per lane
+    double cs_l = 0.0, sn_l = 0.0;
+    auto trig_of = [&](int iv, double& c, double& sn) {
+        if (PIX) {
+            if ((iv & 63) == 0) {
+                const double a = (double)angles[min(iv + (tid & 63), nv - 1)];
+                cs_l = cos(a);
+                sn_l = sin(a);
+            }
+            c = read_lane64(cs_l, iv & 63);
+            sn = read_lane64(sn_l, iv & 63);
+        } else {
+            c = trig[2 * iv];
+            sn = trig[2 * iv + 1];
+        }
+    };
     // the block's voxel-centre box
     const int bx1 = min(bxi * BXS + BXS - 1, nx - 1), by1 = min(byi * BYS + BYS - 1, ny - 1);
     const double cx0 = x0 + (bxi * BXS + 0.5) * vx, cx1 = x0 + (bx1 + 0.5) * vx;
@@ -442,8 +482,8 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
     // are dropped but whose clamped taps are read)
     const double cz0 = z0 + (izc + 0.5) * vz, cz1 = z0 + (izc + SZ - 1 + 0.5) * vz;
     auto footprints = [&](int v0) {
-        return fdk_footprints(trig, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb, inv_da, ai,
-                              inv_db64, bi, na, nb, RW, RH);
+        return fdk_footprints<PIX>(trig, angles, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb,
+                                   inv_da, ai, inv_db64, bi, na, nb, RW, RH);
     };
     // this thread's staging cells: float4 tid + j T of the footprint, row
     // (tid + j T) / R4, cells 4 m .. 4 m + 3 with m = tid % R4 (T is a
@@ -494,7 +534,8 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
         // view iv, from LDS buffer iv & 1 (or from global memory)
         const float* qv = q + (size_t)nb * na * iv;
         const float off = offsets ? offsets[iv] : 0.0f;
-        const double cth = trig[2 * iv], sth = trig[2 * iv + 1];
+        double cth, sth;
+        trig_of(iv, cth, sth);
         // The next view's cells go to the other buffer (nobody reads it before
         // the barrier below) once the first column is done: with several
         // columns per thread their registers are then free for the others.
@@ -508,11 +549,13 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
                 const double iU = rcp64(U);
                 const double mag = SAD * iU;
                 const double fa = ((-x * sth + y * cth) * mag + ha) * inv_da - ai;
-                if (!(fa >= 0.0 && fa <= (double)(na - 1))) break;
+                // PIX: 0 <= a < na - 1 (the op skips a view whose 2 x 2
+                // neighbourhood leaves the detector, pixel.cu.cc:43-50)
+                if (!(fa >= 0.0 && (PIX ? fa < (double)(na - 1) : fa <= (double)(na - 1)))) break;
                 const int ia0 = min((int)floor(fa), na - 2);
                 const float ta = (float)(fa - ia0);
                 const float wr = (float)(SID * iU);
-                const float w = wr * wr;
+                const float w = PIX ? 1.0f : wr * wr;
                 const float mb = (float)mag * inv_db;
                 const float fbv = fmaf(-off, mb, fb0);   // row coordinate of z = 0
                 const float fbs = fmaf(zc, mb, fbv), dzb = no_contract(vzf * mb);
@@ -524,7 +567,8 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
                     for (int k = 0; k < SZ; ++k) {
                         const float fb = fmaf((float)k, dzb, fbs);
                         const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
-                        const bool valid = k < kn && fb == fbc;   // 0 <= fb <= nb - 1
+                        // 0 <= fb <= nb - 1 (PIX: < nb - 1)
+                        const bool valid = k < kn && fb == fbc && (!PIX || fb < nb1);
                         const int ib0 = min((int)fbc, nb - 2);    // fbc >= 0: truncation is floor
                         const float tb = fbc - (float)ib0;
                         const float* p = lb + (__mul24(ib0, SRS) + base);
@@ -540,7 +584,7 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
                     for (int k = 0; k < SZ; ++k) {
                         const float fb = fmaf((float)k, dzb, fbs);
                         const float fbc = __builtin_amdgcn_fmed3f(fb, 0.0f, nb1);
-                        const bool valid = k < kn && fb == fbc;
+                        const bool valid = k < kn && fb == fbc && (!PIX || fb < nb1);
                         const int ib0 = min((int)fbc, nb - 2);
                         const float tb = fbc - (float)ib0;
                         const unsigned o0 = 4u * (unsigned)(__mul24(ib0, na) + ia0), o1 = o0 + rowstep;
@@ -591,6 +635,34 @@ __global__ void k_trig(const float* __restrict__ angles, int nv, double* __restr
 bool finite_pos(double v) { return std::isfinite(v) && v > 0.0; }
 
 }  // namespace
+
+namespace nefct_internal {
+
+bool pixel_bp_staged(const float* pv, const int grid[3], const float center[3], const float size[3],
+                     const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb, float db,
+                     float bi, float* image, hipStream_t stream)
+{
+    const int nx = grid[0], ny = grid[1], nz = grid[2];
+    if (na < 2 || nb < 2 || na >= (1 << 23) || nb >= (1 << 23) || (size_t)na * (size_t)nb >= (1ull << 29))
+        return false;
+    if ((ny + 15) / 16 > 65535 || (nz + SZ - 1) / SZ > 65535) return false;
+    // the op's voxel centres: (i + 0.5) size / n + center
+    const double vx = (double)size[0] / nx, vy = (double)size[1] / ny, vz = (double)size[2] / nz;
+    const double x0 = center[0], y0 = center[1], z0 = center[2];
+    double r2 = 0.0;
+    for (int cxy = 0; cxy < 4; ++cxy) {
+        const double px = x0 + ((cxy & 1) ? nx - 0.5 : 0.5) * vx, py = y0 + ((cxy & 2) ? ny - 0.5 : 0.5) * vy;
+        r2 = std::max(r2, px * px + py * py);
+    }
+    if (!(std::sqrt(r2) <= 0.999 * (double)SID)) return false;
+    const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + SZ - 1) / SZ);
+    hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_SMALL, true>), g, dim3(256), 0, stream, pv, nullptr, nullptr, nv,
+                       nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da,
+                       (double)ai + 0.5, nb, (double)db, (double)bi + 0.5, image, angles);
+    return true;
+}
+
+}  // namespace nefct_internal
 
 extern "C" {
 
