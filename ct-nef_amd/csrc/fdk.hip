@@ -563,6 +563,24 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
                     // float index of row ib0's tap ia0: ib0 SRS + base
                     const float* lb = reinterpret_cast<const float*>(&lds4[iv & 1][0]);
                     const int base = (ia0 - cur.ua0) - cur.vb0 * SRS;
+                    // A column whose SZ row coordinates all lie in [0, nb - 1)
+                    // (fma is monotone in k: its ends bound it) needs no clamp,
+                    // no validity test and no select: the same sums, bitwise.
+                    const float fbl = fmaf((float)(SZ - 1), dzb, fbs);
+                    if (kn == SZ && fminf(fbs, fbl) >= 0.0f && fmaxf(fbs, fbl) < nb1) {
+#pragma unroll
+                        for (int k = 0; k < SZ; ++k) {
+                            const float fb = fmaf((float)k, dzb, fbs);
+                            const int ib0 = (int)fb;                  // 0 <= fb < nb - 1
+                            const float tb = fb - (float)ib0;
+                            const float* p = lb + (__mul24(ib0, SRS) + base);
+                            const float p00 = p[0], p01 = p[1], p10 = p[SRS], p11 = p[SRS + 1];
+                            const float top = fmaf(ta, p01 - p00, p00);
+                            const float bot = fmaf(ta, p11 - p10, p10);
+                            acc[c][k] = fmaf(w, fmaf(tb, bot - top, top), acc[c][k]);
+                        }
+                        break;
+                    }
 #pragma unroll
                     for (int k = 0; k < SZ; ++k) {
                         const float fb = fmaf((float)k, dzb, fbs);
