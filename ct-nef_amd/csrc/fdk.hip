@@ -235,6 +235,28 @@ __global__ __launch_bounds__(256) void k_fdk_bp(const float* __restrict__ q, con
         bool valid[ZC];
         const float fbv = fmaf(-off, mb, fb0);   // row coordinate of z = 0
         const float fbs = fmaf(zc, mb, fbv), dzb = no_contract(vzf * mb);
+        // a full chunk whose row coordinates all lie in [0, nb - 1) (ends
+        // bound a monotone fma): no clamp, test or select, the same sums
+        const float fbl = fmaf((float)(ZC - 1), dzb, fbs);
+        if (PAIR && kn == ZC && fminf(fbs, fbl) >= 0.0f && fmaxf(fbs, fbl) < nb1) {
+            const char* qb = reinterpret_cast<const char*>(qv);
+#pragma unroll
+            for (int k = 0; k < ZC; ++k) {
+                const float fb = fmaf((float)k, dzb, fbs);
+                const int ib0 = (int)fb;
+                tb[k] = fb - (float)ib0;
+                const unsigned o0 = 4u * (unsigned)(__mul24(ib0, na) + ia0), o1 = o0 + rowstep;
+                __builtin_memcpy(&p0[k], qb + o0, 8);
+                __builtin_memcpy(&p1[k], qb + o1, 8);
+            }
+#pragma unroll
+            for (int k = 0; k < ZC; ++k) {
+                const float top = fmaf(ta, p0[k].y - p0[k].x, p0[k].x);
+                const float bot = fmaf(ta, p1[k].y - p1[k].x, p1[k].x);
+                acc[k] = fmaf(w, fmaf(tb[k], bot - top, top), acc[k]);
+            }
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < ZC; ++k) {
             const float fb = fmaf((float)k, dzb, fbs);

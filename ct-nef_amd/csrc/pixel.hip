@@ -106,6 +106,27 @@ __global__ __launch_bounds__(256) void k_pixel_bp(const float* __restrict__ pv, 
             float2 p0[PZ], p1[PZ];
             float wb2[PZ];
             bool valid[PZ];
+            // every voxel of a full chunk on the detector (b is monotone in z,
+            // so its ends bound it): no clamp, no validity test, no select
+            const float bf = fmaf(zc[0], tdb, cb), bl = fmaf(zc[PZ - 1], tdb, cb);
+            if (kn == PZ && fminf(bf, bl) >= 0.0f && fmaxf(bf, bl) < nb1) {
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) {
+                    const float b = fmaf(zc[k], tdb, cb);
+                    const int ib = (int)b;                  // 0 <= b < nb - 1
+                    wb2[k] = b - (float)ib;
+                    const unsigned o0 = __umul24((unsigned)ib, rowstep);
+                    __builtin_memcpy(&p0[k], pview + o0, 8);
+                    __builtin_memcpy(&p1[k], pview + o0 + row1, 8);
+                }
+#pragma unroll
+                for (int k = 0; k < PZ; ++k) {
+                    const float top = fmaf(wa2, p0[k].y - p0[k].x, p0[k].x);
+                    const float bot = fmaf(wa2, p1[k].y - p1[k].x, p1[k].x);
+                    acc[k] = acc[k] + fmaf(wb2[k], bot - top, top);
+                }
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < PZ; ++k) {
                 const float b = fmaf(zc[k], tdb, cb);
