@@ -8,6 +8,8 @@
   between warp inputs, raise before anything is launched.
 * The voxel-driven backprojection on one-row / one-column detectors (every
   view skipped) and beyond its offset limits.
+* The footprint backprojection's direct-add fallbacks (non-finite values,
+  voxels at or behind the source's plane).
 """
 import math
 
@@ -109,3 +111,45 @@ def test_pixel_backproject_degenerate_detectors_and_limits():
                                                   0.0, 4, 1.0, 0.0, img.data_ptr(), None)
     torch.cuda.synchronize()
     assert st == 0
+
+
+def test_footprint_backproject_direct_fallbacks(oracle):
+    """The footprint backprojection sums a workgroup's additions in fixed point
+    only when the scale is defined: a workgroup holding a non-finite value adds
+    directly (the NaN reaches the voxels its pixel touches, as with the
+    reference's float atomics, and nothing else changes), and so does every
+    workgroup when voxels reach the source's plane (checked against the FP64
+    oracle)."""
+    from nefct_hip.fdk import footprint_backproject
+    nx, ny, nz, na, nb = 48, 40, 24, 96, 40
+    kw = dict(SID=300.0, SAD=200.0, da=0.75, ai=0.0, na=na, db=0.75, bi=0.0, nb=nb)
+    ang = torch.tensor([0.2, 1.3], device=DEV)
+    gen = torch.Generator(device=DEV).manual_seed(3)
+    pv = torch.rand((2, nb, na), generator=gen, device=DEV) + 0.5
+    clean = pv.clone()
+    clean[0, 20, 47] = 0.0
+    bad = clean.clone()
+    bad[0, 20, 47] = float("nan")
+    ref = footprint_backproject(clean, (nx, ny, nz), ang, **kw)
+    got = footprint_backproject(bad, (nx, ny, nz), ang, **kw)
+    torch.cuda.synchronize()
+    nan = torch.isnan(got)
+    assert 0 < int(nan.sum()) < 0.01 * got.numel()
+    assert torch.allclose(got[~nan], ref[~nan], rtol=1e-5, atol=1e-5 * ref.abs().max().item())
+    # source 20 voxels from the axis of a 48 x 40 volume, views along the axes:
+    # voxel centres on both sides of the source's plane, none closer than 0.5
+    kw2 = dict(SID=60.0, SAD=20.0, da=0.75, ai=0.0, na=na, db=0.75, bi=0.0, nb=nb)
+    ang2 = np.array([0.0, np.pi / 2], np.float32)
+    p2 = _smooth_projections(2, nb, na)
+    got2 = footprint_backproject(torch.tensor(p2, device=DEV), (nx, ny, nz), torch.tensor(ang2, device=DEV),
+                                 **kw2).cpu().numpy()
+    want = oracle.fdk_fp_bp(0, (nx, ny, nz), ang2.astype(np.float64), None, 60.0, 20.0, na, 0.75, 0.0, nb, 0.75,
+                            0.0, p2.astype(np.float64))
+    assert np.abs(want).max() > 0
+    assert np.linalg.norm(got2 - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def _smooth_projections(nv, nb, na):
+    ib, ia = np.meshgrid(np.arange(nb), np.arange(na), indexing="ij")
+    one = np.exp(-((ia - na / 2) / (na / 5)) ** 2 - ((ib - nb / 2) / (nb / 4)) ** 2)
+    return np.stack([one * (1 + 0.2 * v) for v in range(nv)]).astype(np.float32)
