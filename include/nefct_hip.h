@@ -338,7 +338,7 @@ int nefct_fdk_backproject(const float* filtered, const int grid[3], const float 
  * (what nefct_fdk_backproject does); NEFCT_FDK_BP_GATHER every voxel's taps gathered
  * from global memory; NEFCT_FDK_BP_STAGED each 16x16x16-voxel block's detector
  * footprint staged in LDS once per view; NEFCT_FDK_BP_WIDE the same for
- * 64x64x16-voxel blocks (a third of the HBM traffic, slower).  The staged
+ * 64x64x16-voxel blocks (a fifth of the HBM traffic, slower).  The staged
  * mappings return NEFCT_ERR_UNSUPPORTED unless na, nb >= 2 and every voxel
  * centre lies within 0.999 SID of the rotation axis in-plane.  No reference
  * counterpart (a tuning choice). */
