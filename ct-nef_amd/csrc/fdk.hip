@@ -125,8 +125,8 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
                                                     const float* __restrict__ h, long long rows, int na)
 {
     constexpr int N = 1 << LOGN;
+    constexpr int RPT = (N + 255) / 256;   // response values per thread
     __shared__ float2 buf[2][N];
-    __shared__ float2 resp[N];
     __shared__ float2 tw[N];
     const int tid = threadIdx.x;
     for (int t = tid; t < N; t += 256) {
@@ -145,7 +145,15 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
     }
     __syncthreads();
     const int sr = fft_stockham<N>(buf, 0, tw);
-    for (int i = tid; i < N; i += 256) resp[i] = buf[sr][i];
+    // the response stays in registers: thread tid multiplies entries
+    // tid + 256 j, and LDS holds only the two work buffers and the twiddles
+    // (3 workgroups per CU at N = 2048 instead of 2)
+    float2 resp[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int i = tid + 256 * j;
+        resp[j] = i < N ? buf[sr][i] : make_float2(0.0f, 0.0f);
+    }
     __syncthreads();
     const float inv_n = 1.0f / (float)N;
     for (long long pr = blockIdx.x; 2 * pr < rows; pr += gridDim.x) {
@@ -161,9 +169,13 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
         }
         __syncthreads();
         const int s1 = fft_stockham<N>(buf, 0, tw);
-        for (int i = tid; i < N; i += 256) {   // conj(Z H)
-            const float2 z = buf[s1][i], hh = resp[i];
-            buf[s1][i] = make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {   // conj(Z H)
+            const int i = tid + 256 * j;
+            if (i < N) {
+                const float2 z = buf[s1][i], hh = resp[j];
+                buf[s1][i] = make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+            }
         }
         __syncthreads();
         const int s2 = fft_stockham<N>(buf, s1, tw);
