@@ -79,8 +79,9 @@ __global__ __launch_bounds__(256) void k_fdk_preweight(const float* __restrict__
 constexpr int FMAX = 2048;   // longest detector row supported (N <= 4096)
 
 // Forward DFT of buf[a] (length N = 2^k) by Stockham passes, radix 4 and a
-// last radix-2 pass when k is odd; tw[t] = exp(-2 pi i t / N), t < N.
-// Returns the buffer that holds the result.
+// last radix-2 pass when k is odd; tw[t] = exp(-2 pi i t / N) for t < N / 2
+// (exp(-2 pi i (t + N/2) / N) = -tw[t]; only a radix-4 pass's third twiddle
+// index reaches past N / 2).  Returns the buffer that holds the result.
 template <int N>
 __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
 {
@@ -94,7 +95,11 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
             const float2 v0 = buf[a][j];
             const float2 v1 = cmul(buf[a][j + Q], tw[k * step]);
             const float2 v2 = cmul(buf[a][j + 2 * Q], tw[2 * k * step]);
-            const float2 v3 = cmul(buf[a][j + 3 * Q], tw[3 * k * step]);
+            const int t3 = 3 * k * step;
+            const bool hi = t3 >= N / 2;
+            float2 w3 = tw[hi ? t3 - N / 2 : t3];
+            if (hi) w3 = make_float2(-w3.x, -w3.y);
+            const float2 v3 = cmul(buf[a][j + 3 * Q], w3);
             const int d = ((j - k) << 2) + k;
             buf[a ^ 1][d] = make_float2(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
             buf[a ^ 1][d + Ns] = make_float2(v0.x + v1.y - v2.x - v3.y, v0.y - v1.x - v2.y + v3.x);
@@ -127,9 +132,9 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
     constexpr int N = 1 << LOGN;
     constexpr int RPT = (N + 255) / 256;   // response values per thread
     __shared__ float2 buf[2][N];
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[N / 2];
     const int tid = threadIdx.x;
-    for (int t = tid; t < N; t += 256) {
+    for (int t = tid; t < N / 2; t += 256) {
         float sn, cs;
         sincospif(-2.0f * (float)t / (float)N, &sn, &cs);
         tw[t] = make_float2(cs, sn);
@@ -146,8 +151,8 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
     __syncthreads();
     const int sr = fft_stockham<N>(buf, 0, tw);
     // the response stays in registers: thread tid multiplies entries
-    // tid + 256 j, and LDS holds only the two work buffers and the twiddles
-    // (3 workgroups per CU at N = 2048 instead of 2)
+    // tid + 256 j, and LDS holds only the two work buffers and half the
+    // twiddles (4 workgroups per CU at N = 2048 instead of 2)
     float2 resp[RPT];
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
