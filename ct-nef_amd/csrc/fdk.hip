@@ -78,28 +78,67 @@ __global__ __launch_bounds__(256) void k_fdk_preweight(const float* __restrict__
 // conj(FFT(conj(.))) / N.  FFTs are radix-4 Stockham (self-sorting) in LDS.
 constexpr int FMAX = 2048;   // longest detector row supported (N <= 4096)
 
-// Forward DFT of buf[a] (length N = 2^k) by Stockham passes, radix 4 and a
-// last radix-2 pass when k is odd; tw[t] = exp(-2 pi i t / N) for t < N / 2
-// (exp(-2 pi i (t + N/2) / N) = -tw[t]; only a radix-4 pass's third twiddle
-// index reaches past N / 2).  Returns the buffer that holds the result.
+// Forward DFT of buf[a] (length N = 2^k) by self-sorting Stockham passes:
+// radix 8 while a factor of 8 remains, then one radix-4 or radix-2 pass
+// (2048 = 8 8 8 4: four LDS round trips).  tw[t] = exp(-2 pi i t / N) for
+// t < N / 2; exp(-2 pi i (t + N/2) / N) = -tw[t].  Returns the buffer that
+// holds the result.
+template <int N>
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int t)
+{
+    return t < N / 2 ? tw[t] : make_float2(-tw[t - N / 2].x, -tw[t - N / 2].y);
+}
+
 template <int N>
 __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
 {
     auto cmul = [](float2 u, float2 w) { return make_float2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x); };
-    constexpr int Q = N / 4;
+    auto add = [](float2 u, float2 v) { return make_float2(u.x + v.x, u.y + v.y); };
+    auto sub = [](float2 u, float2 v) { return make_float2(u.x - v.x, u.y - v.y); };
+    auto mi = [](float2 u) { return make_float2(u.y, -u.x); };   // u * (-i)
+    constexpr float R2 = 0.70710678118654752f;
     int Ns = 1;
-    for (; Ns * 4 <= N; Ns *= 4) {
+    for (; Ns * 8 <= N; Ns *= 8) {
+        constexpr int E = N / 8;
+        const int step = N / (Ns * 8);
+        for (int j = threadIdx.x; j < E; j += 256) {
+            const int k = j & (Ns - 1);
+            float2 v[8];
+            v[0] = buf[a][j];
+#pragma unroll
+            for (int m = 1; m < 8; ++m) v[m] = cmul(buf[a][j + m * E], twiddle<N>(tw, m * k * step));
+            // 8-point DFT, decimation in frequency: even outputs from
+            // v_m + v_{m+4}, odd ones from (v_m - v_{m+4}) W8^m
+            const float2 b0 = add(v[0], v[4]), b1 = add(v[1], v[5]), b2 = add(v[2], v[6]), b3 = add(v[3], v[7]);
+            const float2 d0 = sub(v[0], v[4]);
+            const float2 t1 = sub(v[1], v[5]), t3 = sub(v[3], v[7]);
+            const float2 d1 = make_float2(R2 * (t1.x + t1.y), R2 * (t1.y - t1.x));     // W8^1
+            const float2 d2 = mi(sub(v[2], v[6]));                                   // W8^2
+            const float2 d3 = make_float2(R2 * (t3.y - t3.x), -R2 * (t3.x + t3.y));    // W8^3
+            const float2 c0 = add(b0, b2), c2 = sub(b0, b2), c1 = add(b1, b3), c3 = mi(sub(b1, b3));
+            const float2 e0 = add(d0, d2), e2 = sub(d0, d2), e1 = add(d1, d3), e3 = mi(sub(d1, d3));
+            const int d = ((j - k) << 3) + k;
+            buf[a ^ 1][d] = add(c0, c1);
+            buf[a ^ 1][d + 4 * Ns] = sub(c0, c1);
+            buf[a ^ 1][d + 2 * Ns] = add(c2, c3);
+            buf[a ^ 1][d + 6 * Ns] = sub(c2, c3);
+            buf[a ^ 1][d + Ns] = add(e0, e1);
+            buf[a ^ 1][d + 5 * Ns] = sub(e0, e1);
+            buf[a ^ 1][d + 3 * Ns] = add(e2, e3);
+            buf[a ^ 1][d + 7 * Ns] = sub(e2, e3);
+        }
+        __syncthreads();
+        a ^= 1;
+    }
+    if (Ns * 4 == N) {
+        constexpr int Q = N / 4;
         const int step = N / (Ns * 4);
         for (int j = threadIdx.x; j < Q; j += 256) {
             const int k = j & (Ns - 1);
             const float2 v0 = buf[a][j];
-            const float2 v1 = cmul(buf[a][j + Q], tw[k * step]);
-            const float2 v2 = cmul(buf[a][j + 2 * Q], tw[2 * k * step]);
-            const int t3 = 3 * k * step;
-            const bool hi = t3 >= N / 2;
-            float2 w3 = tw[hi ? t3 - N / 2 : t3];
-            if (hi) w3 = make_float2(-w3.x, -w3.y);
-            const float2 v3 = cmul(buf[a][j + 3 * Q], w3);
+            const float2 v1 = cmul(buf[a][j + Q], twiddle<N>(tw, k * step));
+            const float2 v2 = cmul(buf[a][j + 2 * Q], twiddle<N>(tw, 2 * k * step));
+            const float2 v3 = cmul(buf[a][j + 3 * Q], twiddle<N>(tw, 3 * k * step));
             const int d = ((j - k) << 2) + k;
             buf[a ^ 1][d] = make_float2(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
             buf[a ^ 1][d + Ns] = make_float2(v0.x + v1.y - v2.x - v3.y, v0.y - v1.x - v2.y + v3.x);
@@ -108,13 +147,12 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
         }
         __syncthreads();
         a ^= 1;
-    }
-    if (Ns < N) {
+    } else if (Ns * 2 == N) {
         constexpr int H2 = N / 2;
         const int step = N / (Ns * 2);
         for (int j = threadIdx.x; j < H2; j += 256) {
             const int k = j & (Ns - 1);
-            const float2 v0 = buf[a][j], v1 = cmul(buf[a][j + H2], tw[k * step]);
+            const float2 v0 = buf[a][j], v1 = cmul(buf[a][j + H2], twiddle<N>(tw, k * step));
             const int d = ((j - k) << 1) + k;
             buf[a ^ 1][d] = make_float2(v0.x + v1.x, v0.y + v1.y);
             buf[a ^ 1][d + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);

@@ -14,7 +14,8 @@ nefct/cc/distance_3d/dist_3d_back_fdk.cu.cc:154, in the siddon-op geometry
    (filtering.m:20-41,54-88): zero padding to n = max(64, 2^ceil(log2(2 na)))
    makes the FFT filter a linear convolution with taps h[(u - v) mod n]; the
    kernel takes those 2 na - 1 taps and applies them with its own FFT (length
-   the smallest power of two >= 2 na - 1, two rows per complex transform);
+   the smallest power of two >= 2 na - 1, two rows per complex transform,
+   radix-8 Stockham passes in LDS);
 3. every voxel sums (SOD/U)^2 times the bilinear sample of the filtered view at
    its projection (backprojection.m:11-25).
 """
