@@ -144,11 +144,13 @@ def test_fdk_backproject_narrow_detector_partial_z_chunk(oracle, na):
     assert _rel(bp, want) < 1e-5
 
 
-@pytest.mark.parametrize("na,rows", [(1, 3), (2, 1), (37, 5), (64, 4), (333, 7), (1024, 3), (2048, 2)])
+@pytest.mark.parametrize("na,rows", [(1, 3), (2, 1), (37, 5), (64, 4), (100, 3), (200, 5), (333, 7), (1024, 3),
+                                     (2048, 2)])
 def test_fdk_filter_arbitrary_taps(na, rows):
     """nefct_fdk_filter is the linear row convolution with any (2 na - 1) taps
     (centre na - 1): random non-symmetric taps, odd row counts, every FFT
-    length from 64 to 4096, against the direct sum in FP64."""
+    length from 64 to 4096 (radix-8 passes ending in radix 8, 4 or 2), against
+    the direct sum in FP64."""
     from nefct_hip import _lib
     from nefct_hip.ops import _stream_ptr
     rng = np.random.default_rng(na * 31 + rows)
