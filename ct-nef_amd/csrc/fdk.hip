@@ -89,8 +89,13 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int t)
     return t < N / 2 ? tw[t] : make_float2(-tw[t - N / 2].x, -tw[t - N / 2].y);
 }
 
-template <int N>
-__device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
+// With SRC the first pass (radix 8, Ns = 1) takes its inputs from
+// src(u, m, j) = element j + m N/8 (j = threadIdx.x + 256 u) instead of
+// buf[a]; with SNK the last pass hands each output (index, value) to snk
+// instead of storing it.  Both save an LDS round trip and a barrier.
+template <int N, bool SRC = false, bool SNK = false, class Src, class Snk>
+__device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw, Src&& src,
+                                            Snk&& snk)
 {
     auto cmul = [](float2 u, float2 w) { return make_float2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x); };
     auto add = [](float2 u, float2 v) { return make_float2(u.x + v.x, u.y + v.y); };
@@ -101,12 +106,21 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
     for (; Ns * 8 <= N; Ns *= 8) {
         constexpr int E = N / 8;
         const int step = N / (Ns * 8);
-        for (int j = threadIdx.x; j < E; j += 256) {
+        const bool first = Ns == 1, last = Ns * 8 == N;
+#pragma unroll
+        for (int u = 0; u < (E + 255) / 256; ++u) {
+            const int j = threadIdx.x + 256 * u;
+            if (j >= E) break;
             const int k = j & (Ns - 1);
             float2 v[8];
-            v[0] = buf[a][j];
+            if (SRC && first) {
 #pragma unroll
-            for (int m = 1; m < 8; ++m) v[m] = cmul(buf[a][j + m * E], twiddle<N>(tw, m * k * step));
+                for (int m = 0; m < 8; ++m) v[m] = src(u, m, j);   // Ns = 1: every twiddle is 1
+            } else {
+                v[0] = buf[a][j];
+#pragma unroll
+                for (int m = 1; m < 8; ++m) v[m] = cmul(buf[a][j + m * E], twiddle<N>(tw, m * k * step));
+            }
             // 8-point DFT, decimation in frequency: even outputs from
             // v_m + v_{m+4}, odd ones from (v_m - v_{m+4}) W8^m
             const float2 b0 = add(v[0], v[4]), b1 = add(v[1], v[5]), b2 = add(v[2], v[6]), b3 = add(v[3], v[7]);
@@ -117,15 +131,16 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
             const float2 d3 = make_float2(R2 * (t3.y - t3.x), -R2 * (t3.x + t3.y));    // W8^3
             const float2 c0 = add(b0, b2), c2 = sub(b0, b2), c1 = add(b1, b3), c3 = mi(sub(b1, b3));
             const float2 e0 = add(d0, d2), e2 = sub(d0, d2), e1 = add(d1, d3), e3 = mi(sub(d1, d3));
+            const float2 y[8] = {add(c0, c1), add(e0, e1), add(c2, c3), add(e2, e3),
+                                 sub(c0, c1), sub(e0, e1), sub(c2, c3), sub(e2, e3)};
             const int d = ((j - k) << 3) + k;
-            buf[a ^ 1][d] = add(c0, c1);
-            buf[a ^ 1][d + 4 * Ns] = sub(c0, c1);
-            buf[a ^ 1][d + 2 * Ns] = add(c2, c3);
-            buf[a ^ 1][d + 6 * Ns] = sub(c2, c3);
-            buf[a ^ 1][d + Ns] = add(e0, e1);
-            buf[a ^ 1][d + 5 * Ns] = sub(e0, e1);
-            buf[a ^ 1][d + 3 * Ns] = add(e2, e3);
-            buf[a ^ 1][d + 7 * Ns] = sub(e2, e3);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (SNK && last)
+                    snk(d + q * Ns, y[q]);
+                else
+                    buf[a ^ 1][d + q * Ns] = y[q];
+            }
         }
         __syncthreads();
         a ^= 1;
@@ -140,10 +155,17 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
             const float2 v2 = cmul(buf[a][j + 2 * Q], twiddle<N>(tw, 2 * k * step));
             const float2 v3 = cmul(buf[a][j + 3 * Q], twiddle<N>(tw, 3 * k * step));
             const int d = ((j - k) << 2) + k;
-            buf[a ^ 1][d] = make_float2(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y);
-            buf[a ^ 1][d + Ns] = make_float2(v0.x + v1.y - v2.x - v3.y, v0.y - v1.x - v2.y + v3.x);
-            buf[a ^ 1][d + 2 * Ns] = make_float2(v0.x - v1.x + v2.x - v3.x, v0.y - v1.y + v2.y - v3.y);
-            buf[a ^ 1][d + 3 * Ns] = make_float2(v0.x - v1.y - v2.x + v3.y, v0.y + v1.x - v2.y - v3.x);
+            const float2 y[4] = {make_float2(v0.x + v1.x + v2.x + v3.x, v0.y + v1.y + v2.y + v3.y),
+                                 make_float2(v0.x + v1.y - v2.x - v3.y, v0.y - v1.x - v2.y + v3.x),
+                                 make_float2(v0.x - v1.x + v2.x - v3.x, v0.y - v1.y + v2.y - v3.y),
+                                 make_float2(v0.x - v1.y - v2.x + v3.y, v0.y + v1.x - v2.y - v3.x)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (SNK)
+                    snk(d + q * Ns, y[q]);
+                else
+                    buf[a ^ 1][d + q * Ns] = y[q];
+            }
         }
         __syncthreads();
         a ^= 1;
@@ -154,13 +176,27 @@ __device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float
             const int k = j & (Ns - 1);
             const float2 v0 = buf[a][j], v1 = cmul(buf[a][j + H2], twiddle<N>(tw, k * step));
             const int d = ((j - k) << 1) + k;
-            buf[a ^ 1][d] = make_float2(v0.x + v1.x, v0.y + v1.y);
-            buf[a ^ 1][d + Ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
+            const float2 y0 = make_float2(v0.x + v1.x, v0.y + v1.y), y1 = make_float2(v0.x - v1.x, v0.y - v1.y);
+            if (SNK) {
+                snk(d, y0);
+                snk(d + Ns, y1);
+            } else {
+                buf[a ^ 1][d] = y0;
+                buf[a ^ 1][d + Ns] = y1;
+            }
         }
         __syncthreads();
         a ^= 1;
     }
     return a;
+}
+
+template <int N>
+__device__ __forceinline__ int fft_stockham(float2 (*buf)[N], int a, const float2* __restrict__ tw)
+{
+    auto none_src = [](int, int, int) { return make_float2(0.0f, 0.0f); };
+    auto none_snk = [](int, float2) {};
+    return fft_stockham<N, false, false>(buf, a, tw, none_src, none_snk);
 }
 
 template <int LOGN>
@@ -202,30 +238,61 @@ __global__ __launch_bounds__(256) void k_fdk_filter(const float* __restrict__ in
     for (long long pr = blockIdx.x; 2 * pr < rows; pr += gridDim.x) {
         const long long r1 = 2 * pr, r2 = r1 + 1;
         const bool has2 = r2 < rows;
-        for (int i = tid; i < N; i += 256) {
-            float x1 = 0.0f, x2 = 0.0f;
-            if (i < na) {
-                x1 = in[r1 * na + i];
-                if (has2) x2 = in[r2 * na + i];
+        if constexpr (N >= 2048) {
+            // 8 LDS passes per row pair: the forward transform's first pass
+            // reads the rows from global memory (indices >= N/2 > na - 1 are
+            // the zero padding), the inverse's first pass multiplies by the
+            // response held in registers (element tid + 256 (u + m N/2048)),
+            // and its last pass writes the filtered rows out
+            constexpr int E = N / 8;
+            auto load = [&](int, int m, int j) {
+                const int i = j + m * E;
+                float x1 = 0.0f, x2 = 0.0f;
+                if (i < na) {
+                    x1 = in[r1 * na + i];
+                    if (has2) x2 = in[r2 * na + i];
+                }
+                return make_float2(x1, x2);
+            };
+            auto none_snk = [](int, float2) {};
+            const int s1 = fft_stockham<N, true, false>(buf, 0, tw, load, none_snk);
+            auto mulh = [&](int u, int m, int j) {   // conj(Z H)
+                const float2 z = buf[s1][j + m * E], hh = resp[u + m * (E / 256)];
+                return make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+            };
+            auto store = [&](int i, float2 yv) {
+                if (i < na) {
+                    out[r1 * na + i] = yv.x * inv_n;
+                    if (has2) out[r2 * na + i] = -yv.y * inv_n;
+                }
+            };
+            fft_stockham<N, true, true>(buf, s1, tw, mulh, store);
+        } else {
+            for (int i = tid; i < N; i += 256) {
+                float x1 = 0.0f, x2 = 0.0f;
+                if (i < na) {
+                    x1 = in[r1 * na + i];
+                    if (has2) x2 = in[r2 * na + i];
+                }
+                buf[0][i] = make_float2(x1, x2);
             }
-            buf[0][i] = make_float2(x1, x2);
-        }
-        __syncthreads();
-        const int s1 = fft_stockham<N>(buf, 0, tw);
+            __syncthreads();
+            const int s1 = fft_stockham<N>(buf, 0, tw);
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {   // conj(Z H)
-            const int i = tid + 256 * j;
-            if (i < N) {
-                const float2 z = buf[s1][i], hh = resp[j];
-                buf[s1][i] = make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+            for (int j = 0; j < RPT; ++j) {   // conj(Z H)
+                const int i = tid + 256 * j;
+                if (i < N) {
+                    const float2 z = buf[s1][i], hh = resp[j];
+                    buf[s1][i] = make_float2(z.x * hh.x - z.y * hh.y, -(z.x * hh.y + z.y * hh.x));
+                }
             }
-        }
-        __syncthreads();
-        const int s2 = fft_stockham<N>(buf, s1, tw);
-        for (int i = tid; i < na; i += 256) {
-            const float2 y = buf[s2][i];
-            out[r1 * na + i] = y.x * inv_n;
-            if (has2) out[r2 * na + i] = -y.y * inv_n;
+            __syncthreads();
+            const int s2 = fft_stockham<N>(buf, s1, tw);
+            for (int i = tid; i < na; i += 256) {
+                const float2 y = buf[s2][i];
+                out[r1 * na + i] = y.x * inv_n;
+                if (has2) out[r2 * na + i] = -y.y * inv_n;
+            }
         }
         __syncthreads();   // the next pair overwrites buf[0]
     }
