@@ -466,7 +466,7 @@ __device__ __forceinline__ double read_lane64(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <bool PIX>
+template <bool PIX, bool CYL = false>
 __device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __restrict__ angles,
                                const float* __restrict__ offsets, int nv, int v0, double cx0, double cx1,
                                double cy0, double cy1, double cz0, double cz1, double SID, double SAD, double ha,
@@ -489,7 +489,10 @@ __device__ int4 fdk_footprints(const double* __restrict__ trig, const float* __r
     const double U = SID + x * c + y * s;
     const double mag = SAD * rcp64(U);
     // the launcher guarantees U >= 1e-3 SID, so both indices are finite
-    float amn = (float)(((-x * s + y * c) * mag + ha) * inv_da - ai);
+    // CYL: the column is the ray's in-plane angle, monotone in the slope
+    // y'/U, whose extremes over the box lie on its corners as well
+    float amn = CYL ? (float)((double)atan2f((float)(-x * s + y * c), (float)U) * inv_da + ha * inv_da - ai)
+                    : (float)(((-x * s + y * c) * mag + ha) * inv_da - ai);
     float bmn = (float)(((z - off) * mag + hb) * inv_db - bi);
     float amx = amn, bmx = bmn;
 #pragma unroll
@@ -545,7 +548,7 @@ struct StagedShape {
     static_assert(T % R4 == 0, "staging columns the same for every slot");
 };
 
-template <int BXS, int BYS, int NWV, int RW, int RH, bool XCDMAP, bool PIX = false>
+template <int BXS, int BYS, int NWV, int RW, int RH, bool XCDMAP, bool PIX = false, bool CYL = false>
 __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_staged(
     const float* __restrict__ q, const double* __restrict__ trig, const float* __restrict__ offsets, int nv, int nx,
     int ny, int nz, double x0, double y0, double z0, double vx, double vy, double vz, double SID, double SAD, int na,
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
     // are dropped but whose clamped taps are read)
     const double cz0 = z0 + (izc + 0.5) * vz, cz1 = z0 + (izc + SZ - 1 + 0.5) * vz;
     auto footprints = [&](int v0) {
-        return fdk_footprints<PIX>(trig, angles, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb,
+        return fdk_footprints<PIX, CYL>(trig, angles, offsets, nv, v0, cx0, cx1, cy0, cy1, cz0, cz1, SID, SAD, ha, hb,
                                    inv_da, ai, inv_db64, bi, na, nb, RW, RH);
     };
     // this thread's staging cells: float4 tid + j T of the footprint, row
@@ -692,7 +695,11 @@ __global__ __launch_bounds__(64 * NWV, (RW > 64 ? 1 : 16 / NWV)) void k_fdk_bp_s
                 if (!(ix < nx && iy < ny && U > 0.0)) break;
                 const double iU = rcp64(U);
                 const double mag = SAD * iU;
-                const double fa = ((-x * sth + y * cth) * mag + ha) * inv_da - ai;
+                // CYL (cylindrical detector, the voxel-driven op only): the
+                // column from the ray's in-plane angle, atan2 in FP32 as the
+                // op computes it (pixel.cu.cc:70-71)
+                const double fa = CYL ? (double)atan2f((float)(-x * sth + y * cth), (float)U) * inv_da + ha * inv_da - ai
+                                      : ((-x * sth + y * cth) * mag + ha) * inv_da - ai;
                 // PIX: 0 <= a < na - 1 (the op skips a view whose 2 x 2
                 // neighbourhood leaves the detector, pixel.cu.cc:43-50)
                 if (!(fa >= 0.0 && (PIX ? fa < (double)(na - 1) : fa <= (double)(na - 1)))) break;
@@ -800,7 +807,7 @@ bool finite_pos(double v) { return std::isfinite(v) && v > 0.0; }
 
 namespace nefct_internal {
 
-bool pixel_bp_staged(const float* pv, const int grid[3], const float center[3], const float size[3],
+bool pixel_bp_staged(bool cyl, const float* pv, const int grid[3], const float center[3], const float size[3],
                      const float* angles, int nv, float SID, float SAD, int na, float da, float ai, int nb, float db,
                      float bi, float* image, hipStream_t stream)
 {
@@ -818,9 +825,14 @@ bool pixel_bp_staged(const float* pv, const int grid[3], const float center[3], 
     }
     if (!(std::sqrt(r2) <= 0.999 * (double)SID)) return false;
     const dim3 g((nx + 15) / 16, (ny + 15) / 16, (nz + SZ - 1) / SZ);
-    hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_SMALL, true>), g, dim3(256), 0, stream, pv, nullptr, nullptr, nv,
-                       nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da,
-                       (double)ai + 0.5, nb, (double)db, (double)bi + 0.5, image, angles);
+    if (cyl)
+        hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_SMALL, true, true>), g, dim3(256), 0, stream, pv, nullptr,
+                           nullptr, nv, nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da,
+                           (double)ai + 0.5, nb, (double)db, (double)bi + 0.5, image, angles);
+    else
+        hipLaunchKernelGGL((k_fdk_bp_staged<NEFCT_FDK_SMALL, true>), g, dim3(256), 0, stream, pv, nullptr, nullptr,
+                           nv, nx, ny, nz, x0, y0, z0, vx, vy, vz, (double)SID, (double)SAD, na, (double)da,
+                           (double)ai + 0.5, nb, (double)db, (double)bi + 0.5, image, angles);
     return true;
 }
 

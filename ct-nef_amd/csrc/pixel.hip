@@ -15,11 +15,12 @@
 // (-SID, 0, 0) of that frame.  The product is accumulated in FP32 over the
 // views in order, as the reference does (pixel.cu.cc:52).
 //
-// Flat detectors go to the LDS-staged mapping of fdk.hip (16 x 16 x 16 voxel
-// blocks, each view's block footprint copied to LDS once; detector columns in
-// FP64, 1.5x faster at 512 x 512 x 256 voxels and 408 views of 1024 x 768)
-// wherever every voxel centre lies in front of the source; the kernel below
-// takes cylindrical detectors and the rest, with the op's FP32 arithmetic.
+// Calls go to the LDS-staged mapping of fdk.hip (16 x 16 x 16 voxel blocks,
+// each view's block footprint copied to LDS once; detector rows in FP64,
+// flat columns in FP64 and cylindrical ones by FP32 atan2; 1.7x faster at
+// 512 x 512 x 256 voxels and 408 views of 1024 x 768) wherever every voxel
+// centre lies in front of the source; the kernel below takes the rest, with
+// the op's FP32 arithmetic.
 //
 // Layout and launch: one lane per (x, y) column of voxels, a wavefront spans
 // 64 consecutive x, each lane keeps PZ z-values in registers, so the
@@ -170,9 +171,9 @@ int run(bool cyl, const float* pv, const int grid[3], const float center[3], con
     if (!(size[0] > 0 && size[1] > 0 && size[2] > 0 && da > 0 && db > 0 && SID > 0 && SAD > 0))
         return NEFCT_ERR_GEOMETRY;
     hipStream_t s = (hipStream_t)stream;
-    // flat detectors: the LDS-staged mapping where the geometry allows it
-    if (!cyl && nefct_internal::pixel_bp_staged(pv, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi,
-                                                image, s)) {
+    // the LDS-staged mapping where the geometry allows it
+    if (nefct_internal::pixel_bp_staged(cyl, pv, grid, center, size, angles, nv, SID, SAD, na, da, ai, nb, db, bi,
+                                        image, s)) {
         NEFCT_CHECK_LAUNCH();
         return NEFCT_OK;
     }

@@ -361,10 +361,10 @@ int nefct_fdk_backproject_mapped(const float* filtered, const int grid[3], const
  * axis, detector at SAD from the source, index y/d + n/2 (integer n/2)
  * - offset - 0.5.  Host grid/center/size; device projection [nv][nb][na],
  * angles [nv] (radians), image [nz][ny][nx] (fully written).  Limits:
- * 4 na < 2^24, nb < 2^24, 4 na nb < 2^32 (else NEFCT_ERR_SHAPE).  Flat
- * detectors with every voxel centre in front of the source run on the
- * LDS-staged blocks of nefct_fdk_backproject_mapped (detector columns in
- * FP64); cylindrical detectors and the rest use the op's FP32 arithmetic. */
+ * 4 na < 2^24, nb < 2^24, 4 na nb < 2^32 (else NEFCT_ERR_SHAPE).  Volumes
+ * with every voxel centre in front of the source run on the LDS-staged blocks
+ * of nefct_fdk_backproject_mapped (rows and flat columns in FP64, cylindrical
+ * columns by FP32 atan2); the rest use the op's FP32 arithmetic. */
 int nefct_pixel_backproject_flat(const float* projection_value, const int grid[3], const float center[3],
                                  const float size[3], const float* angles, int nv, float SID, float SAD, int na,
                                  float da, float ai, int nb, float db, float bi, float* image, void* stream);
